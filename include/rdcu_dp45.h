@@ -1,0 +1,123 @@
+/*
+ * rdcu_dp45.h -- C ABI of the explicit (Dormand-Prince 4/5, pseudospectral) integrator.
+ *
+ * This is the drop-in boundary for rdcu's non-stiff path.  It replaces the five
+ * module-static calls of the reference integrator
+ *     dp45_init / dp45_step / dp45_run / dp45_eval / dp45_destroy   (dp45.h:7-11)
+ * together with the RHS callback the reference driver plugs into them
+ *     dydt / makeY                                                  (rdcu.cu:117-184)
+ * by handle-based calls with status codes.  The RHS callback becomes a
+ * dictionary descriptor (rdcu_problem) because the RHS is evaluated on the device.
+ *
+ * Two libraries export exactly these symbols:
+ *   rdcu_amd/lib/librdcu_hip.so   -- the MI355X (gfx950) implementation, HIP kernels
+ *   oracle/_bin/liboracle.so      -- the FP64 CPU restatement used as the checker
+ * Plain pointers and sizes only; every array argument is host memory unless a
+ * function name ends in _device.
+ */
+#ifndef RDCU_DP45_H
+#define RDCU_DP45_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference returns nothing and checks no CUDA status
+ * (dp45.cu:87-254); every call here returns one of these. */
+enum {
+  RDCU_OK = 0,
+  RDCU_EINVAL = 1,       /* bad argument (sizes, indices, null pointers)            */
+  RDCU_EUNSUPPORTED = 2, /* valid for the reference but not built here (odd N ...)  */
+  RDCU_EDEVICE = 3,      /* a HIP runtime call failed                               */
+  RDCU_ENOMEM = 4,       /* allocation failed                                       */
+  RDCU_ESTATE = 5,       /* call not valid in the current state (eval before a step) */
+  RDCU_ECALLBACK = 6     /* the step callback asked to stop                         */
+};
+
+/* Problem description: grid, domain and monomial dictionary.
+ * Replaces the fields of the reference's `parameters` struct that the RHS reads
+ * (rdcu.cu:17-76: n, ndim, Ns, Ls, c, nprods, C, nterms).
+ * Term i contributes  coef[i] * prod_j Y[fac[2j]]^fac[2j+1]  to field[i]
+ * (rdcu.cu:175-183); its pairs start at offset 2*sum_{i'<i} nfac[i'] in fac.
+ * Factor index k means Y block k/n of field k%n: block 0 = u, 1+a = d/dx_a,
+ * 1+ndim+a*ndim+b = d/dx_b d/dx_a (rdcu.cu:155-161, 769-785).  Axis 0 is the
+ * slowest-varying axis of the row-major grid. */
+typedef struct rdcu_problem {
+  int n;              /* number of fields (-n)                 */
+  int ndim;           /* 1, 2 or 3 (number of values in -N)    */
+  int Ns[3];          /* grid points per axis (-N)             */
+  double Ls[3];       /* domain length per axis (-L)           */
+  int nterms;         /* number of dictionary terms            */
+  const double *coef; /* [nterms]                              */
+  const int *field;   /* [nterms]                              */
+  const int *nfac;    /* [nterms]                              */
+  const int *fac;     /* [2 * sum(nfac)] (index, power) pairs  */
+} rdcu_problem;
+
+typedef struct rdcu_ctx rdcu_ctx;
+
+/* Creates an integrator for one problem on device `device` (ignored by the CPU
+ * oracle) with initial state y0[n*N] (field-major, row-major grid).
+ * Replaces dp45_init (dp45.h:8, dp45.cu:171-225) and the FFT plan creation
+ * (rdcu.cu:899-900).  fixedstep != 0 selects -F (every attempt accepted). */
+int rdcu_create(const rdcu_problem *prob, double atol, double rtol, int fixedstep,
+                const double *y0, int device, rdcu_ctx **out);
+
+/* Frees everything.  Replaces dp45_destroy (dp45.h:10, dp45.cu:227-254). */
+int rdcu_destroy(rdcu_ctx *ctx);
+
+/* One DP45 attempt from (*t, *h).  *accepted = 1 on acceptance (t and h
+ * advanced, hmax applied), 0 on rejection (h shrunk).  Replaces dp45_step
+ * (dp45.h:7, dp45.cu:87-147); the FSAL copy k1 <- k7 the reference does in
+ * dp45_run (dp45.cu:165-167) is done here after an accepted attempt. */
+int rdcu_step(rdcu_ctx *ctx, double *t, double *h, double hmin, double hmax, int *accepted);
+
+/* Per-attempt callback of rdcu_run, the role of step_eval (rdcu.cu:252-341).
+ * t, h: values after the attempt; success: 1 accepted / 0 rejected.
+ * current: 1 if the integrator still holds this attempt's stages, so that
+ * rdcu_eval and rdcu_get_state refer to it.  *stop: the stop time in force;
+ * the callback may lower or raise it.  Every accepted attempt that reaches
+ * t >= *stop is reported with current == 1.  A nonzero return aborts the run. */
+typedef int (*rdcu_step_fn)(rdcu_ctx *ctx, double t, double h, int success, int current,
+                            double *stop, void *user);
+
+/* Integrates while (*t < t1 && *h > hmin), calling fn after every attempt.
+ * Computes k1 = f(y) first, as dp45_run does (dp45.cu:157).  `stop` is the
+ * first stop time (normally the first output time).  Replaces dp45_run
+ * (dp45.h:9, dp45.cu:154-169). */
+int rdcu_run(rdcu_ctx *ctx, double *t, double *h, double hmin, double hmax, double t1,
+             double stop, rdcu_step_fn fn, void *user);
+
+/* Dense output of the last accepted step at t_eval into y_out[n*N].
+ * Replaces dp45_eval (dp45.h:11, dp45.cu:149-152), which returned a borrowed
+ * device pointer; this copies out. */
+int rdcu_eval(rdcu_ctx *ctx, double t_eval, double *y_out);
+
+/* Copies the current state y into y_out[n*N] (the D2H before fs.dat, rdcu.cu:320). */
+int rdcu_get_state(rdcu_ctx *ctx, double *y_out);
+
+/* Replaces the current state (invalidates the FSAL stage). */
+int rdcu_set_state(rdcu_ctx *ctx, const double *y);
+
+/* Evaluates the dictionary RHS at y[n*N]: f_out[n*N] (may be NULL) and, if
+ * Y_out is not NULL, all n*(1+ndim+ndim^2) factor blocks as complex values
+ * Y_out[2*((blk*n+field)*N + x) + {0,1}] in the layout of FILEBASEY.dat.
+ * Replaces dydt/makeY (rdcu.cu:117-184).  Imaginary parts are those of
+ * Im ifft(m_AH * fft(u)) (the anti-Hermitian Nyquist part of each multiplier);
+ * the identity block's imaginary part is 0. */
+int rdcu_rhs(rdcu_ctx *ctx, const double *y, double *f_out, double *Y_out);
+
+/* Counters: out[0] attempts, out[1] accepted, out[2] RHS evaluations,
+ * out[3] host synchronisations (chunks). */
+int rdcu_get_stats(rdcu_ctx *ctx, long long out[4]);
+
+/* Human-readable message for the last failing call on this thread. */
+const char *rdcu_last_error(void);
+
+/* Name of the backend: "hip-gfx950" or "cpu-oracle". */
+const char *rdcu_backend(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RDCU_DP45_H */
