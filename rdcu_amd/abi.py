@@ -1,0 +1,276 @@
+"""ctypes binding of the integrator C ABI (include/rdcu_dp45.h).
+
+The same binding drives two shared libraries that export identical symbols:
+
+* ``rdcu_amd/lib/librdcu_hip.so`` -- the MI355X implementation (the product);
+* ``oracle/_bin/liboracle.so``   -- the FP64 CPU restatement (test checker only).
+
+``Integrator`` mirrors the reference's integrator module (dp45.h:7-11:
+dp45_init / dp45_step / dp45_run / dp45_eval / dp45_destroy) plus the RHS
+callback the reference driver plugs into it (dydt/makeY, rdcu.cu:117-184),
+with handles instead of module statics and exceptions instead of silent
+failures.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HIP_LIB = os.path.join(ROOT, "rdcu_amd", "lib", "librdcu_hip.so")
+
+# status codes of rdcu_dp45.h
+OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM, ESTATE, ECALLBACK = range(7)
+_ERRNAMES = {1: "EINVAL", 2: "EUNSUPPORTED", 3: "EDEVICE", 4: "ENOMEM", 5: "ESTATE", 6: "ECALLBACK"}
+
+SYMBOLS = (
+    "rdcu_create", "rdcu_destroy", "rdcu_step", "rdcu_run", "rdcu_eval", "rdcu_get_state",
+    "rdcu_set_state", "rdcu_rhs", "rdcu_get_stats", "rdcu_last_error", "rdcu_backend",
+)
+
+
+class RdcuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Problem(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("ndim", ctypes.c_int), ("Ns", ctypes.c_int * 3),
+        ("Ls", ctypes.c_double * 3), ("nterms", ctypes.c_int),
+        ("coef", ctypes.POINTER(ctypes.c_double)), ("field", ctypes.POINTER(ctypes.c_int)),
+        ("nfac", ctypes.POINTER(ctypes.c_int)), ("fac", ctypes.POINTER(ctypes.c_int)),
+    ]
+
+
+_DP = ctypes.POINTER(ctypes.c_double)
+STEP_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                           ctypes.c_int, ctypes.c_int, _DP, ctypes.c_void_p)
+
+_loaded = {}
+
+
+def load(path: str) -> ctypes.CDLL:
+    """Loads a library exporting the C ABI and declares its signatures."""
+    path = os.path.abspath(path)
+    if path in _loaded:
+        return _loaded[path]
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    lib.rdcu_create.argtypes = [ctypes.POINTER(_Problem), ctypes.c_double, ctypes.c_double,
+                                ctypes.c_int, _DP, ctypes.c_int, ctypes.POINTER(vp)]
+    lib.rdcu_destroy.argtypes = [vp]
+    lib.rdcu_step.argtypes = [vp, _DP, _DP, ctypes.c_double, ctypes.c_double,
+                              ctypes.POINTER(ctypes.c_int)]
+    lib.rdcu_run.argtypes = [vp, _DP, _DP, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_double, STEP_FN, vp]
+    lib.rdcu_eval.argtypes = [vp, ctypes.c_double, _DP]
+    lib.rdcu_get_state.argtypes = [vp, _DP]
+    lib.rdcu_set_state.argtypes = [vp, _DP]
+    lib.rdcu_rhs.argtypes = [vp, _DP, _DP, _DP]
+    lib.rdcu_get_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+    lib.rdcu_last_error.restype = ctypes.c_char_p
+    lib.rdcu_backend.restype = ctypes.c_char_p
+    for s in SYMBOLS:
+        if s not in ("rdcu_last_error", "rdcu_backend"):
+            getattr(lib, s).restype = ctypes.c_int
+    _loaded[path] = lib
+    return lib
+
+
+def hip_lib() -> ctypes.CDLL:
+    """The MI355X library.  Raises if it is not built: there is no fallback."""
+    return load(HIP_LIB)
+
+
+@dataclass
+class Term:
+    """One dictionary term ``coef * prod_k Y[idx_k]**pow_k`` added to ``field``.
+
+    Same meaning as one ``-c C,field,idx,pow,...`` argument of rdcu
+    (rdcu.cu:654-661, help text rdcu.cu:740-745)."""
+    coef: float
+    field: int
+    factors: List[Tuple[int, int]] = field(default_factory=list)
+
+    @staticmethod
+    def parse(text: str) -> "Term":
+        tok = [t for t in text.split(",") if t != ""]
+        ints = [int(float(t)) for t in tok[1:]]
+        f = ints[0] if ints else 0
+        rest = ints[1:]
+        facs = [(rest[2 * j], rest[2 * j + 1] if 2 * j + 1 < len(rest) else 0)
+                for j in range((len(rest) + 1) // 2)]
+        return Term(float(tok[0]), f, facs)
+
+
+@dataclass
+class Problem:
+    """Grid, domain and dictionary (the RHS part of rdcu.cu's ``parameters``)."""
+    n: int
+    Ns: Sequence[int]
+    Ls: Sequence[float]
+    terms: List[Term]
+
+    @property
+    def ndim(self) -> int:
+        return len(self.Ns)
+
+    @property
+    def N(self) -> int:
+        return int(np.prod(self.Ns))
+
+    @property
+    def nblk(self) -> int:
+        d = self.ndim
+        return 1 + d + d * d
+
+    def index(self, block: int, field: int) -> int:
+        """Factor index of (block, field): block 0 = u, 1+a = d/dx_a,
+        1+ndim+a*ndim+b = d/dx_b d/dx_a (rdcu.cu:769-785)."""
+        return block * self.n + field
+
+    def _c(self):
+        p = _Problem()
+        p.n = self.n
+        p.ndim = self.ndim
+        for a in range(3):
+            p.Ns[a] = int(self.Ns[a]) if a < self.ndim else 1
+            p.Ls[a] = float(self.Ls[a]) if a < self.ndim else 1.0
+        p.nterms = len(self.terms)
+        coef = np.array([t.coef for t in self.terms] or [0.0], dtype=np.float64)
+        fld = np.array([t.field for t in self.terms] or [0], dtype=np.int32)
+        nfac = np.array([len(t.factors) for t in self.terms] or [0], dtype=np.int32)
+        fac = np.array([v for t in self.terms for pair in t.factors for v in pair] or [0],
+                       dtype=np.int32)
+        keep = (coef, fld, nfac, fac)
+        p.coef = coef.ctypes.data_as(_DP)
+        p.field = fld.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        p.nfac = nfac.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        p.fac = fac.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        return p, keep
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_DP)
+
+
+class Integrator:
+    """Handle-based DP45 integrator over one library (dp45.h:7-11)."""
+
+    def __init__(self, lib: ctypes.CDLL, prob: Problem, y0: np.ndarray, atol=1e-8, rtol=1e-8,
+                 fixed=False, device=0):
+        self.lib = lib
+        self.prob = prob
+        self.nN = prob.n * prob.N
+        y0 = np.ascontiguousarray(y0, dtype=np.float64).reshape(-1)
+        if y0.size != self.nN:
+            raise ValueError(f"y0 has {y0.size} values, expected {self.nN}")
+        p, keep = prob._c()
+        h = ctypes.c_void_p()
+        self._check(lib.rdcu_create(ctypes.byref(p), atol, rtol, int(bool(fixed)), _ptr(y0),
+                                    device, ctypes.byref(h)))
+        del keep
+        self.h = h
+
+    def _check(self, code):
+        if code != OK:
+            raise RdcuError(code, self.lib.rdcu_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rdcu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, t: float, h: float, hmin=0.0, hmax=np.inf):
+        """One attempt (dp45_step, dp45.cu:87-147): returns (t, h, accepted)."""
+        tt, hh, acc = ctypes.c_double(t), ctypes.c_double(h), ctypes.c_int()
+        self._check(self.lib.rdcu_step(self.h, ctypes.byref(tt), ctypes.byref(hh), hmin, hmax,
+                                       ctypes.byref(acc)))
+        return tt.value, hh.value, bool(acc.value)
+
+    def run(self, t: float, h: float, t1: float, hmin=0.0, hmax=np.inf, stop=np.inf,
+            callback: Optional[Callable] = None):
+        """dp45_run (dp45.cu:154-169).  callback(t, h, success, current) may
+        return a new stop time (or None to keep it).  Returns (t, h)."""
+        err = []
+
+        def cb(ctx, t, h, ok, cur, stopp, user):
+            try:
+                if callback is not None:
+                    r = callback(t, h, bool(ok), bool(cur))
+                    if r is not None:
+                        stopp[0] = float(r)
+                return 0
+            except Exception as e:  # surface Python errors after the run
+                err.append(e)
+                return 1
+
+        fn = STEP_FN(cb)
+        tt, hh = ctypes.c_double(t), ctypes.c_double(h)
+        code = self.lib.rdcu_run(self.h, ctypes.byref(tt), ctypes.byref(hh), hmin, hmax, t1,
+                                 stop, fn, None)
+        if err:
+            raise err[0]
+        self._check(code)
+        return tt.value, hh.value
+
+    def eval(self, t_eval: float) -> np.ndarray:
+        out = np.empty(self.nN)
+        self._check(self.lib.rdcu_eval(self.h, t_eval, _ptr(out)))
+        return out
+
+    def state(self) -> np.ndarray:
+        out = np.empty(self.nN)
+        self._check(self.lib.rdcu_get_state(self.h, _ptr(out)))
+        return out
+
+    def set_state(self, y: np.ndarray):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+        self._check(self.lib.rdcu_set_state(self.h, _ptr(y)))
+
+    def rhs(self, y: np.ndarray, factors=False):
+        """f = dydt(y); with factors=True also every Y block as complex
+        [nblk, n, N] in the layout of FILEBASEY.dat."""
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+        f = np.empty(self.nN)
+        Y = np.empty(2 * self.nN * self.prob.nblk) if factors else None
+        self._check(self.lib.rdcu_rhs(self.h, _ptr(y), _ptr(f),
+                                      _ptr(Y) if factors else None))
+        if factors:
+            return f, Y.view(np.complex128).reshape(self.prob.nblk, self.prob.n, self.prob.N)
+        return f
+
+    def stats(self):
+        out = (ctypes.c_longlong * 4)()
+        self._check(self.lib.rdcu_get_stats(self.h, out))
+        return dict(attempts=out[0], accepted=out[1], rhs=out[2], syncs=out[3])
+
+
+def glibc_rand_ic(n: int, N: int, seed: int = 1, A: float = 1e-3, A0: float = 0.0) -> np.ndarray:
+    """Random initial condition of rdcu.cu:898, 991-1001:
+    y = A0 + A/2*(2.0/RAND_MAX*rand()-1) after srand(seed), field-major."""
+    libc = ctypes.CDLL("libc.so.6")
+    libc.rand.restype = ctypes.c_int
+    libc.srand(ctypes.c_uint(seed))
+    r = np.fromiter((libc.rand() for _ in range(n * N)), dtype=np.float64, count=n * N)
+    return A0 + A / 2 * (2.0 / 2147483647 * r - 1)

@@ -1,0 +1,186 @@
+// rd_fft.h -- FP64 complex arithmetic and the LDS Stockham FFT used by every
+// axis pass of the pseudospectral RHS (gfx950, wave64).
+//
+// The operation sequence of every butterfly, twiddle product and real-split
+// step is fixed (explicit fma, the file is compiled with -ffp-contract=off) so
+// that results are reproducible bit for bit on any grid tiling and batch
+// layout.  Twiddles come from host tables T_L[m] = exp(-2 pi i m / L).
+//
+// Radix plan for a length L = 5^c 3^b 2^a: c radix-5 passes, b radix-3, one
+// radix-2 (a%3==1) or radix-4 (a%3==2), then a/3 radix-8 passes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace rd {
+
+struct __align__(16) cx {
+  double re, im;
+};
+
+__device__ __forceinline__ cx cadd(cx a, cx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cx csub(cx a, cx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cx cconj(cx a) { return {a.re, -a.im}; }
+__device__ __forceinline__ cx cmul(cx a, cx w) {
+  return {__builtin_fma(a.re, w.re, -(a.im * w.im)), __builtin_fma(a.re, w.im, a.im * w.re)};
+}
+// times -i (DIR < 0, forward) or +i (DIR > 0, inverse)
+template <int DIR>
+__device__ __forceinline__ cx crot(cx a) {
+  if constexpr (DIR < 0) return {a.im, -a.re};
+  else return {-a.im, a.re};
+}
+
+constexpr double kR2 = 0x1.6a09e667f3bcdp-1;   // sqrt(1/2)
+constexpr double kS3 = 0x1.bb67ae8584caap-1;   // sqrt(3)/2
+constexpr double kC51 = 0x1.3c6ef372fe950p-2;  // cos(2pi/5)
+constexpr double kC52 = -0x1.9e3779b97f4a8p-1; // cos(4pi/5)
+constexpr double kS51 = 0x1.e6f0e134454ffp-1;  // sin(2pi/5)
+constexpr double kS52 = 0x1.2cf2304755a5ep-1;  // sin(4pi/5)
+
+template <int DIR>
+__device__ __forceinline__ cx crot8_1(cx a) {
+  if constexpr (DIR < 0) return {(a.re + a.im) * kR2, (a.im - a.re) * kR2};
+  else return {(a.re - a.im) * kR2, (a.im + a.re) * kR2};
+}
+template <int DIR>
+__device__ __forceinline__ cx crot8_3(cx a) {
+  if constexpr (DIR < 0) return {(a.im - a.re) * kR2, -((a.re + a.im) * kR2)};
+  else return {-((a.re + a.im) * kR2), (a.re - a.im) * kR2};
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(cx &x0, cx &x1, cx &x2, cx &x3) {
+  cx a = cadd(x0, x2), b = csub(x0, x2);
+  cx c = cadd(x1, x3), d = crot<DIR>(csub(x1, x3));
+  x0 = cadd(a, c);
+  x1 = cadd(b, d);
+  x2 = csub(a, c);
+  x3 = csub(b, d);
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft(cx *v) {
+  if constexpr (R == 2) {
+    cx a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    dft4<DIR>(v[0], v[1], v[2], v[3]);
+  } else if constexpr (R == 8) {
+    cx e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    cx o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<DIR>(e0, e1, e2, e3);
+    dft4<DIR>(o0, o1, o2, o3);
+    cx t1 = crot8_1<DIR>(o1), t2 = crot<DIR>(o2), t3 = crot8_3<DIR>(o3);
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, t1);
+    v[5] = csub(e1, t1);
+    v[2] = cadd(e2, t2);
+    v[6] = csub(e2, t2);
+    v[3] = cadd(e3, t3);
+    v[7] = csub(e3, t3);
+  } else if constexpr (R == 3) {
+    cx t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    cx y0 = cadd(v[0], t);
+    cx m = {v[0].re - 0.5 * t.re, v[0].im - 0.5 * t.im};
+    cx u = crot<DIR>(cx{kS3 * d.re, kS3 * d.im});
+    v[0] = y0;
+    v[1] = cadd(m, u);
+    v[2] = csub(m, u);
+  } else if constexpr (R == 5) {
+    cx t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    cx d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+    cx y0 = cadd(v[0], cadd(t1, t2));
+    cx a1 = {__builtin_fma(kC52, t2.re, __builtin_fma(kC51, t1.re, v[0].re)),
+             __builtin_fma(kC52, t2.im, __builtin_fma(kC51, t1.im, v[0].im))};
+    cx a2 = {__builtin_fma(kC51, t2.re, __builtin_fma(kC52, t1.re, v[0].re)),
+             __builtin_fma(kC51, t2.im, __builtin_fma(kC52, t1.im, v[0].im))};
+    cx b1 = {__builtin_fma(kS51, d1.re, kS52 * d2.re), __builtin_fma(kS51, d1.im, kS52 * d2.im)};
+    cx b2 = {__builtin_fma(kS52, d1.re, -(kS51 * d2.re)),
+             __builtin_fma(kS52, d1.im, -(kS51 * d2.im))};
+    cx r1 = crot<DIR>(b1), r2 = crot<DIR>(b2);
+    v[0] = y0;
+    v[1] = cadd(a1, r1);
+    v[4] = csub(a1, r1);
+    v[2] = cadd(a2, r2);
+    v[3] = csub(a2, r2);
+  }
+}
+
+constexpr int kMaxPass = 12;
+
+// Radix plan of one transform length, passed by value in kernel arguments.
+struct FftPlan {
+  int L;
+  int np;
+  int rad[kMaxPass];
+  const cx *tw;  // T_L, L entries, device memory
+};
+
+// One Stockham pass over `nl` lines in LDS.  Element e of line l lives at
+// buf[l*ls + e*es].  LFAST selects whether consecutive threads walk lines
+// (interleaved column tiles) or butterflies (contiguous rows).
+template <int R, int DIR, bool LFAST>
+__device__ __forceinline__ void stockham_pass(const cx *__restrict__ src, cx *__restrict__ dst,
+                                              int L, int Ns, const cx *__restrict__ tw, int nl,
+                                              int ls, int es, int tid, int nt) {
+  const int stride = L / R;
+  const int twstep = L / (Ns * R);
+  const int nb = nl * stride;
+  for (int w = tid; w < nb; w += nt) {
+    int l, j;
+    if constexpr (LFAST) {
+      l = w % nl;
+      j = w / nl;
+    } else {
+      l = w / stride;
+      j = w - l * stride;
+    }
+    const int k = j % Ns;
+    const cx *s = src + l * ls;
+    cx v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = s[(j + r * stride) * es];
+    if (k != 0) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cx wv = tw[(k * r) * twstep];
+        if constexpr (DIR > 0) wv.im = -wv.im;
+        v[r] = cmul(v[r], wv);
+      }
+    }
+    dft<R, DIR>(v);
+    cx *d = dst + l * ls;
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[(base + r * Ns) * es] = v[r];
+  }
+}
+
+// Full transform of nl lines held in LDS buffer a (b is scratch).  Every
+// thread of the block must call it; it ends with a barrier and returns the
+// buffer holding the result.
+template <int DIR, bool LFAST>
+__device__ cx *lds_fft(cx *a, cx *b, const FftPlan &p, int nl, int ls, int es, int tid, int nt) {
+  cx *src = a, *dst = b;
+  int Ns = 1;
+  for (int ip = 0; ip < p.np; ++ip) {
+    const int R = p.rad[ip];
+    switch (R) {
+      case 8: stockham_pass<8, DIR, LFAST>(src, dst, p.L, Ns, p.tw, nl, ls, es, tid, nt); break;
+      case 4: stockham_pass<4, DIR, LFAST>(src, dst, p.L, Ns, p.tw, nl, ls, es, tid, nt); break;
+      case 2: stockham_pass<2, DIR, LFAST>(src, dst, p.L, Ns, p.tw, nl, ls, es, tid, nt); break;
+      case 3: stockham_pass<3, DIR, LFAST>(src, dst, p.L, Ns, p.tw, nl, ls, es, tid, nt); break;
+      case 5: stockham_pass<5, DIR, LFAST>(src, dst, p.L, Ns, p.tw, nl, ls, es, tid, nt); break;
+    }
+    __syncthreads();
+    Ns *= R;
+    cx *t = src;
+    src = dst;
+    dst = t;
+  }
+  return src;
+}
+
+}  // namespace rd

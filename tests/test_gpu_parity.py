@@ -1,0 +1,178 @@
+"""GPU parity: the HIP library against the CPU oracle through the same C ABI.
+
+Both follow one operation sequence (radix plan, twiddle tables, fma
+placement, reduction trees, correctly rounded pow), so results are compared
+for exact equality (== on doubles), not with a tolerance.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from rdcu_amd import Integrator, Problem, Term, cgle_terms, fhn_terms, glibc_rand_ic
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_terms(ndim):
+    """Every derivative block of both fields appears (first derivatives and
+    mixed second derivatives exercise all Nyquist rules)."""
+    n = 2
+    nblk = 1 + ndim + ndim * ndim
+    t = []
+    for blk in range(1, nblk):
+        for f in range(n):
+            t.append(Term(0.5 + 0.1 * blk, f, [(blk * n + f, 1), ((blk * n + 1 - f) % (n * nblk), 1)]))
+    t.append(Term(-1.0, 0, [(0, 3)]))
+    t.append(Term(0.3, 1, [(0, 2), (1, -1)]))
+    t.append(Term(0.7, 0, [(1, 0)]))
+    t.append(Term(1.25, 1, []))
+    return t
+
+
+RHS_CASES = [
+    ("1d-256", [256], [100.0], "cgle"),
+    ("1d-96", [96], [30.0], "mixed"),
+    ("2d-64", [64, 64], [50.0, 50.0], "cgle"),
+    ("2d-64x96", [64, 96], [40.0, 70.0], "mixed"),
+    ("2d-96x64", [96, 64], [70.0, 40.0], "fhn"),
+    ("2d-192x128", [192, 128], [150.0, 90.0], "cgle"),
+    ("2d-512", [512, 512], [200.0, 200.0], "cgle"),
+    ("3d-16x24x32", [16, 24, 32], [10.0, 20.0, 30.0], "mixed"),
+    ("3d-32", [32, 32, 32], [30.0, 30.0, 30.0], "cgle"),
+]
+
+
+def make_problem(Ns, Ls, kind):
+    nd = len(Ns)
+    if kind == "cgle":
+        terms = cgle_terms(nd)
+    elif kind == "fhn":
+        terms = fhn_terms(nd, 4.0, 0.05, 1.2)
+    else:
+        terms = mixed_terms(nd)
+    return Problem(2, Ns, Ls, terms)
+
+
+@pytest.mark.parametrize("name,Ns,Ls,kind", RHS_CASES, ids=[c[0] for c in RHS_CASES])
+def test_rhs_and_factors_bitwise(name, Ns, Ls, kind, hlib, olib, rng):
+    prob = make_problem(Ns, Ls, kind)
+    y = rng.uniform(-1.0, 1.0, prob.n * prob.N)
+    if kind == "mixed":
+        y += 2.0  # keep away from zero (negative power)
+    print(f"case {name}", flush=True)
+    with Integrator(hlib, prob, y) as g, Integrator(olib, prob, y) as o:
+        fg, Yg = g.rhs(y, factors=True)
+        fo, Yo = o.rhs(y, factors=True)
+    assert np.array_equal(Yg.real, Yo.real), np.abs(Yg.real - Yo.real).max()
+    assert np.array_equal(fg, fo), np.abs(fg - fo).max()
+
+
+def test_rhs_plain_equals_factor_path(hlib, rng):
+    """f from the referenced-factor pipeline equals f recomputed with the
+    full factor set materialised (same kernels, different factor list)."""
+    prob = make_problem([64, 96], [40.0, 70.0], "mixed")
+    y = rng.uniform(1.0, 2.0, prob.n * prob.N)
+    with Integrator(hlib, prob, y) as g:
+        f1 = g.rhs(y)
+        f2, _ = g.rhs(y, factors=True)
+        f3 = g.rhs(y)
+    assert np.array_equal(f1, f2) and np.array_equal(f1, f3)
+
+
+STEP_CASES = [
+    ("1d-256", [256], [100.0], "cgle", 1.0),
+    ("2d-64", [64, 64], [50.0, 50.0], "cgle", 1.0),
+    ("2d-96x64-fhn", [96, 64], [70.0, 40.0], "fhn", 1.0),
+    ("3d-16x24x32", [16, 24, 32], [10.0, 20.0, 30.0], "cgle", 1.0),
+]
+
+
+@pytest.mark.parametrize("name,Ns,Ls,kind,A", STEP_CASES, ids=[c[0] for c in STEP_CASES])
+def test_step_sequence_bitwise(name, Ns, Ls, kind, A, hlib, olib):
+    """rdcu_step attempts (accepted and rejected) give identical t, h, y."""
+    prob = make_problem(Ns, Ls, kind)
+    y0 = glibc_rand_ic(prob.n, prob.N, seed=1, A=A)
+    print(f"case {name}", flush=True)
+    with Integrator(hlib, prob, y0) as g, Integrator(olib, prob, y0) as o:
+        tg = to = 0.0
+        hg = ho = 0.01
+        nacc = nrej = 0
+        for i in range(25):
+            tg, hg, ag = g.step(tg, hg)
+            to, ho, ao = o.step(to, ho)
+            assert (tg, hg, ag) == (to, ho, ao), (i, tg, to, hg, ho, ag, ao)
+            nacc += ag
+            nrej += not ag
+        assert np.array_equal(g.state(), o.state())
+        if ag:
+            assert np.array_equal(g.eval(tg - 0.3 * hg), o.eval(to - 0.3 * ho))
+    assert nacc > 5
+
+
+def test_run_with_outputs_bitwise(hlib, olib):
+    """rdcu_run with output stops (device-side halting, chunked replay) gives
+    the same accepted/rejected sequence and dense outputs as the oracle."""
+    prob = Problem(2, [256], [100.0], cgle_terms(1))
+    y0 = glibc_rand_ic(2, 256, seed=1, A=1.0)
+    res = {}
+    for name, lib in (("gpu", hlib), ("cpu", olib)):
+        with Integrator(lib, prob, y0) as it:
+            log = []
+            outs = []
+            nxt = [1.0]
+
+            def cb(t, h, ok, cur):
+                log.append((t, h, ok))
+                while ok and t >= nxt[0] and nxt[0] <= 10.0:
+                    assert cur
+                    outs.append(it.eval(nxt[0]))
+                    nxt[0] += 1.0
+                return nxt[0]
+
+            t, h = it.run(0.0, 0.01, 10.0, stop=1.0, callback=cb)
+            res[name] = (log, outs, t, h, it.state())
+    lg, og, tg, hg, yg = res["gpu"]
+    lc, oc, tc, hc, yc = res["cpu"]
+    assert len(lg) == len(lc)
+    assert lg == lc
+    assert len(og) == len(oc) == 10
+    for a, b in zip(og, oc):
+        assert np.array_equal(a, b)
+    assert (tg, hg) == (tc, hc)
+    assert np.array_equal(yg, yc)
+
+
+def test_fixed_step_bitwise(hlib, olib):
+    prob = Problem(2, [64, 64], [50.0, 50.0], cgle_terms(2))
+    y0 = glibc_rand_ic(2, 64 * 64, seed=3, A=1.0)
+    out = []
+    for lib in (hlib, olib):
+        with Integrator(lib, prob, y0, fixed=True) as it:
+            t, h = it.run(0.0, 0.01, 0.2)
+            out.append((t, h, it.state()))
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1] == 0.01
+    assert np.array_equal(out[0][2], out[1][2])
+
+
+def _run_cli(exe, args, cwd):
+    r = subprocess.run([exe] + args, cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r
+
+
+def test_cli_outputs_identical(tmp_path):
+    """The drop-in CLI on the GPU writes byte-identical output files to the
+    same CLI on the oracle (1-D CGLE, -D3)."""
+    from conftest import HIP_CLI, ORACLE_CLI
+    args = ["-n", "2", "-N", "256", "-L", "100", "-t", "5", "-d", "0.5", "-A", "1", "-D3"]
+    args += sum((["-c", c] for c in __import__("rdcu_amd").to_cli(cgle_terms(1))), [])
+    for sub, exe in (("g", HIP_CLI), ("c", ORACLE_CLI)):
+        d = tmp_path / sub
+        d.mkdir()
+        _run_cli(exe, args + ["run"], str(d))
+    for suf in ("steps.dat", "times.dat", "states.dat", "f.dat", "Y.dat", "fs.dat"):
+        a = (tmp_path / "g" / ("run" + suf)).read_bytes()
+        b = (tmp_path / "c" / ("run" + suf)).read_bytes()
+        assert a == b, suf
