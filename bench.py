@@ -1,0 +1,145 @@
+#!/usr/bin/env python3
+"""Benchmark: DP45 steps/s on the 2-D complex Ginzburg-Landau problem, 512x512,
+FP64, on one MI355X, with the RHS evaluation measured against the HBM roofline.
+
+Workload (synthetic, as the reference's own runs): two real fields u0 = Re A,
+u1 = Im A of A_t = A + (1+2i) lap A - (1-0.8i)|A|^2 A (the 18-term dictionary
+of the reference's README example), periodic 512x512 grid, L = 200x200,
+random initial condition of amplitude 1 from glibc rand() with seed 1
+(rdcu -A 1 -s 1), rtol = atol = 1e-8.  A "step" is one DP45 attempt (six RHS
+evaluations plus the error norm and step control, all on the device); the
+state and every intermediate stay in HBM for the whole timed region.
+
+Prints one JSON line.  Diagnostics go to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(n, M, Ns):
+    """Bytes one RHS evaluation must move through HBM, per kernel, for
+    half-spectrum storage (8 B per grid point per field or factor) with one
+    read and one write per array per pass:
+      x forward   read u (8nN), write n half spectra (8nN)
+      axis 0      read n spectra (8nN), write M factor spectra (8MN)
+      x inverse   read M factor spectra (8MN) and u (8nN), write f (8nN)
+    (2-D; the sum is B_rhs = 8N[n(2*ndim+1) + 2M(ndim-1)])."""
+    N = 1
+    for s in Ns:
+        N *= s
+    fwd, col, inv = 16 * n * N, 8 * n * N + 8 * M * N, 8 * M * N + 16 * n * N
+    return {"row_r2c": fwd, "row_fwd": fwd, "col_mult": col, "row_c2r": inv, "row_inv": inv}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600, help="timed DP45 attempts")
+    ap.add_argument("--warmup", type=int, default=60, help="untimed attempts first")
+    ap.add_argument("--grid", type=int, default=512, help="grid points per axis")
+    ap.add_argument("--reps", type=int, default=200, help="RHS evaluations per kernel timing")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="wall-clock budget of the CPU-oracle sample (0 to skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each instance lives "
+              "on one GPU; see DESIGN.md for how a run would shard). Use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from rdcu_amd import Integrator, Problem, cgle_terms, glibc_rand_ic, hip_lib, load
+
+    G = args.grid
+    Ns, Ls = [G, G], [200.0, 200.0]
+    prob = Problem(2, Ns, Ls, cgle_terms(2, 2.0, 0.8))
+    y0 = glibc_rand_ic(2, G * G, seed=1, A=1.0)
+    lib = hip_lib()
+    it = Integrator(lib, prob, y0, atol=1e-8, rtol=1e-8)
+
+    t, h = 0.0, 0.01  # -d 1: first step dt/100
+    t, h, done = it.advance(t, h, max_attempts=args.warmup)
+    s0 = it.stats()
+    w0 = time.perf_counter()
+    t, h, done = it.advance(t, h, max_attempts=args.steps)  # returns after a stream sync
+    w1 = time.perf_counter()
+    s1 = it.stats()
+    el = w1 - w0
+    attempts = s1["attempts"] - s0["attempts"]
+    accepted = s1["accepted"] - s0["accepted"]
+    log(f"timed: {attempts} attempts ({accepted} accepted) in {el:.4f} s, t={t:.6f} h={h:.6f}")
+
+    # RHS kernels, HIP events on the library's stream
+    km = it.time_rhs(args.reps)
+    log("kernel ms:", json.dumps(km))
+    M = 4  # referenced derivative factors: d2/dy2 and d2/dx2 of both fields
+    kb = {k: v for k, v in algorithmic_bytes(2, M, Ns).items() if k in km}
+    B_rhs = sum(kb.values())
+    dom = max((k for k in km if k != "rhs"), key=lambda k: km[k])
+    ach = kb[dom] / (km[dom] * 1e-3)
+    rhs_ach = B_rhs / (km["rhs"] * 1e-3)
+
+    result = {
+        "metric": "DP45 steps/s, 2D CGLE 512² FP64 (%HBM roofline); batched vs serial solves/s",
+        "value": accepted / el,
+        "unit": "accepted DP45 steps/s",
+        "n_gpus": 1,
+        "steps": attempts,
+        "warmup": args.warmup,
+        "ms_per_step": el / attempts * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (glibc rand() initial condition, seed 1, amplitude 1)",
+        "config": {"workload": f"2-D CGLE {G}x{G}, L=200x200, b=2, c=0.8, 2 FP64 fields, "
+                               "18-term dictionary, rtol=atol=1e-8, DP45 pseudospectral",
+                   "grid": Ns, "attempts_per_s": attempts / el, "accepted": accepted,
+                   "t_end": t, "h_end": h, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": None,
+                     "bytes_per_launch": kb[dom], "ms_per_launch": km[dom]},
+        "rhs": {"ms": km["rhs"], "bytes": B_rhs, "achieved_GBps": rhs_ach / 1e9,
+                "frac": rhs_ach / PEAK_HBM, "kernels_ms": km, "kernel_bytes": kb,
+                "rhs_per_s": 1e3 / km["rhs"]},
+    }
+
+    if args.cpu_seconds > 0:
+        import ctypes
+        ora = load(os.path.join(ROOT, "oracle", "_bin", "liboracle.so"))
+        ora.oracle_get_threads.restype = ctypes.c_int
+        cores = ora.oracle_get_threads()
+        with Integrator(ora, prob, y0, atol=1e-8, rtol=1e-8) as o:
+            tc, hc, na, nacc = 0.0, 0.01, 0, 0
+            c0 = time.perf_counter()
+            while time.perf_counter() - c0 < args.cpu_seconds or na < 2:
+                tc, hc, ok = o.step(tc, hc)
+                na += 1
+                nacc += ok
+            cel = time.perf_counter() - c0
+        result["cpu_baseline"] = {
+            "value": nacc / cel, "unit": "accepted DP45 steps/s", "cores": cores, "kind": "port",
+            "sample": f"first {na} DP45 attempts ({nacc} accepted) of the same 512x512 CGLE "
+                      f"run from the same initial condition, FP64 C++ oracle, OpenMP "
+                      f"{cores} threads, {cel:.1f} s",
+            "attempts_per_s": na / cel}
+        log("cpu:", json.dumps(result["cpu_baseline"]))
+
+    it.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -88,6 +88,22 @@ typedef int (*rdcu_step_fn)(rdcu_ctx *ctx, double t, double h, int success, int 
 int rdcu_run(rdcu_ctx *ctx, double *t, double *h, double hmin, double hmax, double t1,
              double stop, rdcu_step_fn fn, void *user);
 
+/* Runs attempts while (*t < t1 && *h > hmin), at most max_attempts of them,
+ * with no per-attempt callback (a run without outputs, e.g. a burn-in or a
+ * benchmark).  *done receives the number of attempts run.  Same loop and step
+ * control as rdcu_run (dp45.cu:154-169). */
+int rdcu_advance(rdcu_ctx *ctx, double *t, double *h, double hmin, double hmax, double t1,
+                 long long max_attempts, long long *done);
+
+/* Timing of the RHS pipeline on the library's stream with HIP events.
+ * ms[0]: one RHS evaluation of the current state (all its kernels), averaged
+ * over `reps` back-to-back evaluations; ms[1..count-1]: each kernel of that
+ * pipeline launched alone `reps` times, averaged.  `names` receives the
+ * comma-separated kernel names ("rhs,<k1>,<k2>,...").  The CPU oracle fills
+ * ms[0] only (wall clock). */
+int rdcu_time_rhs(rdcu_ctx *ctx, int reps, double *ms, int max, int *count, char *names,
+                  int names_len);
+
 /* Dense output of the last accepted step at t_eval into y_out[n*N].
  * Replaces dp45_eval (dp45.h:11, dp45.cu:149-152), which returned a borrowed
  * device pointer; this copies out. */
@@ -102,9 +118,9 @@ int rdcu_set_state(rdcu_ctx *ctx, const double *y);
 /* Evaluates the dictionary RHS at y[n*N]: f_out[n*N] (may be NULL) and, if
  * Y_out is not NULL, all n*(1+ndim+ndim^2) factor blocks as complex values
  * Y_out[2*((blk*n+field)*N + x) + {0,1}] in the layout of FILEBASEY.dat.
- * Replaces dydt/makeY (rdcu.cu:117-184).  Imaginary parts are those of
- * Im ifft(m_AH * fft(u)) (the anti-Hermitian Nyquist part of each multiplier);
- * the identity block's imaginary part is 0. */
+ * Replaces dydt/makeY (rdcu.cu:117-184).  Imaginary parts are written as 0
+ * (the reference leaves the anti-Hermitian Nyquist leakage there, and
+ * uninitialised memory in block 0). */
 int rdcu_rhs(rdcu_ctx *ctx, const double *y, double *f_out, double *Y_out);
 
 /* Counters: out[0] attempts, out[1] accepted, out[2] RHS evaluations,

@@ -810,6 +810,34 @@ int rdcu_run(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double
   return RDCU_OK;
 }
 
+int rdcu_advance(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double t1,
+                 long long max_attempts, long long *done) {
+  if (!c || !t || !h) return fail(RDCU_EINVAL, "null argument");
+  long long k = 0;
+  c->E.rhs(c->y.data(), c->k[0].data());
+  c->k1_valid = true;
+  while (k < max_attempts && *t < t1 && *h > hmin) {
+    int acc = 0;
+    int st = attempt(c, t, h, hmax, &acc);
+    if (st) return st;
+    ++k;
+  }
+  if (done) *done = k;
+  return RDCU_OK;
+}
+
+int rdcu_time_rhs(rdcu_ctx *c, int reps, double *ms, int max, int *count, char *names,
+                  int names_len) {
+  if (!c || !ms || max < 1 || reps < 1) return fail(RDCU_EINVAL, "bad argument");
+  std::vector<double> f(c->y.size());
+  double t0 = omp_get_wtime();
+  for (int r = 0; r < reps; ++r) c->E.rhs(c->y.data(), f.data());
+  ms[0] = (omp_get_wtime() - t0) * 1e3 / reps;
+  if (count) *count = 1;
+  if (names && names_len > 0) snprintf(names, names_len, "rhs");
+  return RDCU_OK;
+}
+
 // y_eval = y_old + hh*(x Q0 + x^2 Q1 + x^3 Q2 + x^4 Q3) (dp45.cu:70-76, 149-152)
 int rdcu_eval(rdcu_ctx *c, double t_eval, double *y_out) {
   if (!c || !y_out) return fail(RDCU_EINVAL, "null argument");

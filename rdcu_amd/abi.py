@@ -28,8 +28,9 @@ OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM, ESTATE, ECALLBACK = range(7)
 _ERRNAMES = {1: "EINVAL", 2: "EUNSUPPORTED", 3: "EDEVICE", 4: "ENOMEM", 5: "ESTATE", 6: "ECALLBACK"}
 
 SYMBOLS = (
-    "rdcu_create", "rdcu_destroy", "rdcu_step", "rdcu_run", "rdcu_eval", "rdcu_get_state",
-    "rdcu_set_state", "rdcu_rhs", "rdcu_get_stats", "rdcu_last_error", "rdcu_backend",
+    "rdcu_create", "rdcu_destroy", "rdcu_step", "rdcu_run", "rdcu_advance", "rdcu_time_rhs",
+    "rdcu_eval", "rdcu_get_state", "rdcu_set_state", "rdcu_rhs", "rdcu_get_stats",
+    "rdcu_last_error", "rdcu_backend",
 )
 
 
@@ -71,6 +72,10 @@ def load(path: str) -> ctypes.CDLL:
                               ctypes.POINTER(ctypes.c_int)]
     lib.rdcu_run.argtypes = [vp, _DP, _DP, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                              ctypes.c_double, STEP_FN, vp]
+    lib.rdcu_advance.argtypes = [vp, _DP, _DP, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)]
+    lib.rdcu_time_rhs.argtypes = [vp, ctypes.c_int, _DP, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                  ctypes.c_char_p, ctypes.c_int]
     lib.rdcu_eval.argtypes = [vp, ctypes.c_double, _DP]
     lib.rdcu_get_state.argtypes = [vp, _DP]
     lib.rdcu_set_state.argtypes = [vp, _DP]
@@ -233,6 +238,22 @@ class Integrator:
             raise err[0]
         self._check(code)
         return tt.value, hh.value
+
+    def advance(self, t: float, h: float, t1=np.inf, max_attempts=1, hmin=0.0, hmax=np.inf):
+        """Attempts without per-attempt callback; returns (t, h, attempts run)."""
+        tt, hh, done = ctypes.c_double(t), ctypes.c_double(h), ctypes.c_longlong()
+        self._check(self.lib.rdcu_advance(self.h, ctypes.byref(tt), ctypes.byref(hh), hmin, hmax,
+                                          t1, int(max_attempts), ctypes.byref(done)))
+        return tt.value, hh.value, done.value
+
+    def time_rhs(self, reps=100):
+        """{kernel name: average ms} from HIP events ("rhs" = whole evaluation)."""
+        ms = (ctypes.c_double * 16)()
+        cnt = ctypes.c_int()
+        names = ctypes.create_string_buffer(512)
+        self._check(self.lib.rdcu_time_rhs(self.h, int(reps), ms, 16, ctypes.byref(cnt), names,
+                                           512))
+        return dict(zip(names.value.decode().split(","), list(ms[:cnt.value])))
 
     def eval(self, t_eval: float) -> np.ndarray:
         out = np.empty(self.nN)

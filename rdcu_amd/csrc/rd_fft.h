@@ -9,7 +9,9 @@
 // Radix plan for a length L = 5^c 3^b 2^a: c radix-5 passes, b radix-3, one
 // radix-2 (a%3==1) or radix-4 (a%3==2), then a/3 radix-8 passes.
 #pragma once
+#ifndef __HIPCC_RTC__  // hiprtc (run-time specialised kernels) has the HIP builtins already
 #include <hip/hip_runtime.h>
+#endif
 
 namespace rd {
 
@@ -181,6 +183,83 @@ __device__ cx *lds_fft(cx *a, cx *b, const FftPlan &p, int nl, int ls, int es, i
     dst = t;
   }
   return src;
+}
+
+// ------------------------------------------------------------------------
+// Register-resident Stockham.  A line of length L is held by T = L/E threads;
+// thread t keeps elements t + T*q (q < E) in registers.  Pass p (radix R,
+// Ns = product of the earlier radices) is, for butterfly j = t + T*b
+// (b < E/R): inputs j + (L/R)*r = t + T*(b + (E/R)*r) -- always the thread's
+// own registers -- then the same twiddle/butterfly arithmetic as
+// stockham_pass, outputs to LDS at (j-k)*R + k + r*Ns.  The outputs of the
+// last pass land at j + r*Ns = the thread's own slots, so the result stays in
+// registers.  E must be a multiple of every radix of the plan.
+// LDS slot of element e in the register path: one pad slot per 8 elements,
+// so the stride-R scatter of the early passes spreads over all banks.
+__device__ __forceinline__ int pad8(int e) { return e + (e >> 3); }
+constexpr int padded_len(int L) { return L + (L >> 3); }
+
+template <int R, int E, int DIR>
+__device__ __forceinline__ void reg_pass(cx (&v)[E], cx *buf, int es, const cx *tw, int L, int Ns,
+                                         int t, int T, bool last) {
+  constexpr int B = E / R;
+  const int twstep = L / (Ns * R);
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = t + T * b;
+    const int k = j % Ns;
+    cx u[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) u[r] = v[b + B * r];
+    if (k != 0) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cx w = tw[(k * r) * twstep];
+        if constexpr (DIR > 0) w.im = -w.im;
+        u[r] = cmul(u[r], w);
+      }
+    }
+    dft<R, DIR>(u);
+    if (last) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + B * r] = u[r];
+    } else {
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[pad8(base + r * Ns) * es] = u[r];
+    }
+  }
+}
+
+// Full transform of the line whose element e lives at buf[pad8(e)*es] (LDS
+// scratch of padded_len(L) elements); input and output in v.  Every thread of the block calls it
+// (it contains barriers); on return the scratch may be reused.
+template <int E, int DIR>
+__device__ void reg_fft(cx (&v)[E], cx *buf, int es, const cx *tw, const FftPlan &p, int t, int T) {
+  int Ns = 1;
+  for (int ip = 0; ip < p.np; ++ip) {
+    const int R = p.rad[ip];
+    if (ip > 0) {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = buf[pad8(t + T * q) * es];
+      __syncthreads();
+    }
+    const bool last = ip == p.np - 1;
+    if constexpr (E % 8 == 0) {
+      if (R == 8) { reg_pass<8, E, DIR>(v, buf, es, tw, p.L, Ns, t, T, last); Ns *= R; continue; }
+    }
+    if constexpr (E % 4 == 0) {
+      if (R == 4) { reg_pass<4, E, DIR>(v, buf, es, tw, p.L, Ns, t, T, last); Ns *= R; continue; }
+    }
+    if constexpr (E % 3 == 0) {
+      if (R == 3) { reg_pass<3, E, DIR>(v, buf, es, tw, p.L, Ns, t, T, last); Ns *= R; continue; }
+    }
+    if constexpr (E % 2 == 0) {
+      if (R == 2) { reg_pass<2, E, DIR>(v, buf, es, tw, p.L, Ns, t, T, last); Ns *= R; continue; }
+    }
+    Ns *= R;  // unreachable for plans the host validated
+  }
 }
 
 }  // namespace rd

@@ -276,6 +276,7 @@ struct ColGeom {
   int ntc;     // column tiles per group
   int tc;      // columns per tile
   int axis;    // grid axis of the lines (multiplier index)
+  int nparts;  // blocks per tile in mode 1 (factor split, register path)
 };
 
 __global__ void k_col(Grid g, FftPlan p, ColGeom cg, DevState *st, int check_active, Bufs b,

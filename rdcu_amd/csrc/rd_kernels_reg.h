@@ -1,0 +1,537 @@
+// rd_kernels_reg.h -- register-resident variants of the axis passes.
+//
+// Same arithmetic as rd_kernels.h operation for operation (same radix plan,
+// twiddles, real-split steps, multipliers, dictionary, stage sums and error
+// tree), so results are bit-identical; only the data movement differs: each
+// thread keeps E elements of a line in VGPRs across the Stockham passes, LDS
+// carries only the transposes between passes, twiddles are staged in LDS.
+//
+// k_rowi_reg also fuses the next DP45 stage: after it has produced k_s for a
+// row it forms ytemp_{s+1} = y + h*sum_j a_j k_j for that row and transforms
+// it, so one RHS is two launches (k_col_reg, k_rowi_reg) instead of three.
+#pragma once
+#include "rd_kernels.h"
+
+namespace rd {
+
+// Dictionary program of the register path: int32 words, staged into LDS.
+//   words [0, n]: start of field f's term list (word n: end of the program)
+//   term record : nfac, coef low word, coef high word, then nfac (slot, pow)
+// A field's terms keep their input order.  Same meaning as Dict.
+struct DictP {
+  const int *prog;  // device memory
+  int nwords;
+};
+
+// Field f of the dictionary at XM points x0 + stride*i of one row (value rows
+// in LDS, vals[slot*Nx + x]): terms of field f in input order, each a product
+// of integer powers started from 1, accumulated with fma from 0 -- the
+// per-field sequence of operations of rdcu.cu:175-183.
+template <int XM>
+__device__ __forceinline__ void dict_field(const int *P, int f, const double *vals, int Nx, int x0,
+                                           int stride, double (&acc)[XM]) {
+#pragma unroll
+  for (int i = 0; i < XM; ++i) acc[i] = 0.0;
+  int p = P[f];
+  const int pe = P[f + 1];
+  while (p < pe) {
+    const int nfac = P[p];
+    const double cc = __hiloint2double(P[p + 2], P[p + 1]);
+    p += 3;
+    double term[XM];
+#pragma unroll
+    for (int i = 0; i < XM; ++i) term[i] = 1.0;
+    for (int j = 0; j < nfac; ++j, p += 2) {
+      const double *vr = vals + P[p] * Nx;
+      const int pw = P[p + 1];
+#pragma unroll
+      for (int i = 0; i < XM; ++i) {
+        const int x = x0 + stride * i;
+        const double v = x < Nx ? vr[x] : 0.0;
+        term[i] = term[i] * ipow(v, pw);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < XM; ++i) acc[i] = __builtin_fma(cc, term[i], acc[i]);
+  }
+}
+
+// Dictionary evaluator reading the program from LDS (any dictionary).  The
+// runtime can replace it by straight-line code generated for one dictionary
+// (same operations in the same order).
+struct ProgDict {
+  static constexpr bool kProg = true;
+  template <int XM>
+  __device__ __forceinline__ static void eval(const int *P, int f, const double *vals, int Nx,
+                                              int x0, int stride, double (&acc)[XM]) {
+    dict_field<XM>(P, f, vals, Nx, x0, stride, acc);
+  }
+};
+
+// Copies n complex values global -> LDS with U loads in flight per thread
+// before the first store (a plain strided loop waits out one memory latency
+// per element).
+template <int U>
+__device__ __forceinline__ void stage_cx(cx *dst, const cx *__restrict__ src, int n, int tid,
+                                         int nt) {
+  for (int i0 = tid; i0 < n; i0 += U * nt) {
+    cx tmp[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * nt;
+      tmp[u] = i < n ? src[i] : cx{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * nt;
+      if (i < n) dst[i] = tmp[u];
+    }
+  }
+}
+
+template <int U, class V>
+__device__ __forceinline__ void stage_v(V *dst, const V *__restrict__ src, int n, int tid, int nt) {
+  for (int i0 = tid; i0 < n; i0 += U * nt) {
+    V tmp[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * nt;
+      tmp[u] = i < n ? src[i] : V{};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * nt;
+      if (i < n) dst[i] = tmp[u];
+    }
+  }
+}
+
+// Multiplier value of derivative block blk at wavenumber indices nk with the
+// wavenumbers of each axis already loaded (kv) -- the same operations as
+// apply_mult.
+__device__ __forceinline__ cx mult_k(const Grid &g, int blk, const int *nk, const double *kv, cx X) {
+  const int nd = g.ndim;
+  if (blk <= nd) {
+    const int a = blk - 1;
+    if (nk[a] == g.Ns[a] / 2) return {0.0, 0.0};
+    const double val = kv[a] * g.invN;
+    return {-(X.im * val), X.re * val};
+  }
+  const int q = blk - 1 - nd;
+  const int a = q / nd, b = q - a * nd;
+  double val;
+  if (a == b) {
+    val = (kv[a] * kv[a]) * g.invN;
+  } else {
+    const bool na = nk[a] == g.Ns[a] / 2, nb = nk[b] == g.Ns[b] / 2;
+    if (na != nb) return {0.0, 0.0};
+    val = (kv[a] * kv[b]) * g.invN;
+  }
+  return {-(X.re * val), -(X.im * val)};
+}
+
+// Forward x pass of rows (plain input or DP45 stage sum) -> S.
+// Block: rpb rows x T threads (T = L/E).  LDS: twiddles L + rpb*padded_len(L)
+// complex.
+template <int E>
+__global__ __launch_bounds__(256) void k_rowf_reg(Grid g, FftPlan px, const cx *__restrict__ twN, DevState *st,
+                           int check_active, Bufs b, int src_kind, const double *src_ptr,
+                           StageCoef sc, int rpb) {
+  extern __shared__ cx smem[];
+  if (check_active && !attempt_active(st)) return;
+  const int L = g.L, Nx = g.Nx, Nxh = g.Nxh, T = L / E;
+  const long nrows = (long)g.n * g.rows;
+  cx *tw = smem, *buf = smem + L;
+  const int tid = threadIdx.x;
+  const int rr = tid / T, t = tid - rr * T;
+  const long row = (long)blockIdx.x * rpb + rr;
+  const bool act = row < nrows;
+  stage_cx<8>(tw, px.tw, L, tid, blockDim.x);
+
+  cx v[E];
+  if (act) {
+    if (src_kind == SRC_STAGE) {
+      const double h = st->h;
+      const double *y = b.y[st->iy];
+      double *yt = b.y[st->iyt];
+      const double *ks[6];
+      ks[0] = b.k[st->k1s];
+      for (int j = 1; j < 6; ++j) ks[j] = b.k[j];
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const long off = row * Nx + 2 * (t + T * q);
+        const double2 yv = *reinterpret_cast<const double2 *>(y + off);
+        const double2 k1 = *reinterpret_cast<const double2 *>(ks[0] + off);
+        double ar = sc.a[0] * k1.x, ai = sc.a[0] * k1.y;
+        for (int s = 1; s < sc.ns; ++s) {
+          if (sc.a[s] == 0.0) continue;
+          const double2 kv = *reinterpret_cast<const double2 *>(ks[s] + off);
+          ar = __builtin_fma(sc.a[s], kv.x, ar);
+          ai = __builtin_fma(sc.a[s], kv.y, ai);
+        }
+        double2 o;
+        o.x = __builtin_fma(h, ar, yv.x);
+        o.y = __builtin_fma(h, ai, yv.y);
+        *reinterpret_cast<double2 *>(yt + off) = o;
+        v[q] = cx{o.x, o.y};
+      }
+    } else {
+      const double *src =
+          src_kind == SRC_Y ? b.y[st->iy] : (src_kind == SRC_YT ? b.y[st->iyt] : src_ptr);
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const double2 o = *reinterpret_cast<const double2 *>(src + row * Nx + 2 * (t + T * q));
+        v[q] = cx{o.x, o.y};
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cx{0.0, 0.0};
+  }
+  cx *lb = buf + rr * padded_len(L);
+  reg_fft<E, -1>(v, lb, 1, tw, px, t, T);
+#pragma unroll
+  for (int q = 0; q < E; ++q) lb[t + T * q] = v[q];
+  __syncthreads();
+  if (!act) return;
+  cx *S = b.S + row * Nxh;
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int k = t + T * q;
+    const cx Av = v[q];
+    const cx Bv = cconj(lb[k == 0 ? 0 : L - k]);
+    const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
+    const cx tt = cmul(D, twN[k]);
+    S[k] = cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)};
+  }
+  if (t == 0) {
+    const cx Av = lb[0];
+    const cx Bv = cconj(lb[0]);
+    const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
+    const cx tt = cmul(D, twN[L]);
+    S[L] = cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)};
+  }
+}
+
+// Strided-line pass (modes as k_col).  Block: tc columns x T threads; LDS:
+// twiddles L + tc*padded_len(L) complex (element e of column c at
+// buf[pad8(e)*tc + c]) + tc*L complex holding the forward spectrum (mode 1).
+template <int E>
+__global__ __launch_bounds__(256) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st, int check_active, Bufs b,
+                          FacList fl, int mode, int fpw) {
+  extern __shared__ cx smem[];
+  if (check_active && !attempt_active(st)) return;
+  // mode 1: blocks come in groups of nparts, each transforming up to fpw of
+  // the field's factors (the forward transform is repeated per part)
+  const int nparts = mode == 1 ? cg.nparts : 1;
+  const int part = blockIdx.x % nparts;
+  const int tile = blockIdx.x / nparts;
+  const int grp = tile / cg.ntc, ct = tile - grp * cg.ntc;
+  const int g1 = grp / cg.gdiv, g2 = grp - g1 * cg.gdiv;
+  int m0 = 0, m1 = 0;
+  if (mode == 1) {
+    m0 = fl.fstart[g1] + part * fpw;
+    m1 = fl.fstart[g1 + 1];
+    if (m1 > m0 + fpw) m1 = m0 + fpw;
+    if (m0 >= m1) return;
+  }
+  const int L = cg.L, T = L / E, tc = cg.tc;
+  const int c0 = ct * tc;
+  const int nc = (g.Nxh - c0) < tc ? (g.Nxh - c0) : tc;
+  const int tid = threadIdx.x;
+  const int c = tid % tc, t = tid / tc;
+  const bool act = c < nc;
+  cx *tw = smem, *buf = smem + L;
+  stage_cx<8>(tw, p.tw, L, tid, blockDim.x);
+  // mode 1: wavenumbers of axis 0 after the forward spectrum in LDS
+  double *k0s = reinterpret_cast<double *>(buf + tc * padded_len(L) + tc * L);
+  if (mode == 1) stage_v<8>(k0s, g.kt[0], L, tid, blockDim.x);
+  const long lbase = (long)g2 * cg.gs2 + c0 + c;
+  cx *src = (mode == 2 ? b.F : b.S) + (long)g1 * cg.gs1 + lbase;
+  cx v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = act ? src[(long)(t + T * q) * cg.es] : cx{0.0, 0.0};
+  cx *lb = buf + c;
+  if (mode != 1) {
+    if (mode == 0) reg_fft<E, -1>(v, lb, tc, tw, p, t, T);
+    else reg_fft<E, +1>(v, lb, tc, tw, p, t, T);
+    if (act) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) src[(long)(t + T * q) * cg.es] = v[q];
+    }
+    return;
+  }
+  reg_fft<E, -1>(v, lb, tc, tw, p, t, T);
+  // keep the forward spectrum in LDS (each thread reloads only its own
+  // elements, so no barrier is needed) instead of registers across factors
+  cx *xs = buf + tc * padded_len(L);
+#pragma unroll
+  for (int q = 0; q < E; ++q) xs[(t + T * q) * tc + c] = v[q];
+  const int nd = g.ndim;
+  double kv[3];
+  int nk[3];
+  const int kx = c0 + (act ? c : 0);
+  if (nd == 2) {
+    nk[1] = kx;
+    kv[1] = g.kt[1][kx];
+  } else {
+    nk[1] = g2;
+    kv[1] = g.kt[1][g2];
+    nk[2] = kx;
+    kv[2] = g.kt[2][kx];
+  }
+  for (int m = m0; m < m1; ++m) {
+    const int blk = fl.fblk[m];
+    cx w[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      nk[0] = t + T * q;
+      kv[0] = k0s[t + T * q];
+      w[q] = mult_k(g, blk, nk, kv, xs[(t + T * q) * tc + c]);
+    }
+    reg_fft<E, +1>(w, lb, tc, tw, p, t, T);
+    if (act) {
+      cx *dst = b.F + (long)m * g.Nh + lbase;
+#pragma unroll
+      for (int q = 0; q < E; ++q) dst[(long)(t + T * q) * cg.es] = w[q];
+    }
+  }
+}
+
+// Inverse x pass for one row position r, all fields: factor lines in groups
+// of G (thread (gl, t) owns line g0+gl), dictionary, stage-7 error tree, and
+// with nx.ns > 0 the fused next stage (stage sum + forward x pass -> S).
+// Block: G*T threads, G >= n.  LDS: twiddles L + G*padded_len(L) complex,
+// then (n+nf)*Nx + n*Nx doubles, then the dictionary program.
+template <int E, class DE>
+__device__ __forceinline__ void rowi_body(Grid g, FftPlan px, const cx *__restrict__ twN,
+                                          DevState *st, int check_active, Bufs b, FacList fl,
+                                          DictP dict, int src_kind, const double *src_ptr,
+                                          int dst_kind, int dst_slot, double *dst_ptr, int err,
+                                          ErrCoef ec, StageCoef nx, int G) {
+  extern __shared__ cx smem[];
+  if (check_active && !attempt_active(st)) return;
+  const int n = g.n, Nx = g.Nx, L = g.L, Nxh = g.Nxh, T = L / E;
+  const long N = g.N;
+  const long r = blockIdx.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int gl = tid / T, t = tid - gl * T;
+  const int nf = fl.nf;
+  const int PL = padded_len(L);
+  cx *tw = smem, *buf = smem + L;
+  double *vals = reinterpret_cast<double *>(buf + G * PL);
+  double *fv = vals + (long)(n + nf) * Nx;
+  int *prog = reinterpret_cast<int *>(fv + (long)n * Nx);
+  cx *lb = buf + gl * PL;
+  stage_cx<8>(tw, px.tw, L, tid, nt);
+  if constexpr (DE::kProg) stage_v<8>(prog, dict.prog, dict.nwords, tid, nt);
+
+  const double *src =
+      src_kind == SRC_Y ? b.y[st->iy] : (src_kind == SRC_YT ? b.y[st->iyt] : src_ptr);
+  for (int x0 = tid; x0 < Nx; x0 += 4 * nt) {  // identity rows: all loads in flight first
+    double tmp[kNMax][4];
+#pragma unroll
+    for (int f = 0; f < kNMax; ++f)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = x0 + i * nt;
+        tmp[f][i] = (f < n && x < Nx) ? src[f * N + r * Nx + x] : 0.0;
+      }
+#pragma unroll
+    for (int f = 0; f < kNMax; ++f)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = x0 + i * nt;
+        if (f < n && x < Nx) vals[f * Nx + x] = tmp[f][i];
+      }
+  }
+  for (int g0 = 0; g0 < nf; g0 += G) {
+    const int m = g0 + gl;
+    const bool act = m < nf;
+    cx v[E];
+    if (act) {
+      const int blk = fl.fblk[m];
+      const cx *row = g.ndim == 1 ? b.S + (long)fl.ffield[m] * g.Nh : b.F + (long)m * g.Nh + r * Nxh;
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const int k = t + T * q;
+        cx Av = row[k], Bv = row[L - k];
+        if (g.ndim == 1) {
+          int nk0 = k, nk1 = L - k;
+          Av = apply_mult(g, blk, &nk0, Av);
+          Bv = apply_mult(g, blk, &nk1, Bv);
+        }
+        Bv = cconj(Bv);
+        if (k == 0) {
+          Av.im = 0.0;
+          Bv.im = 0.0;
+        }
+        const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
+        const cx tt = cmul(D, cconj(twN[k]));
+        v[q] = cx{Ev.re - tt.im, Ev.im + tt.re};
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = cx{0.0, 0.0};
+    }
+    reg_fft<E, +1>(v, lb, 1, tw, px, t, T);
+    if (act) {
+      double *vr = vals + (long)(n + m) * Nx;
+#pragma unroll
+      for (int q = 0; q < E; ++q)
+        *reinterpret_cast<double2 *>(vr + 2 * (t + T * q)) = double2{v[q].re, v[q].im};
+    }
+  }
+  __syncthreads();
+
+  double *out;
+  if (dst_kind == DST_PTR) out = dst_ptr;
+  else if (dst_kind == DST_K1) out = b.k[st->k1s];
+  else if (dst_kind == DST_K7) out = b.k[st->k7s];
+  else out = b.k[dst_slot];
+  double *sq = reinterpret_cast<double *>(buf);  // G*L complex >= n*Nx doubles
+  const double *yv = nullptr, *k1 = nullptr;
+  double h = 0, atol = 0, rtol = 0;
+  if (err) {
+    yv = b.y[st->iy];
+    k1 = b.k[st->k1s];
+    h = st->h;
+    atol = st->atol;
+    rtol = st->rtol;
+  }
+  constexpr int XM = 4;
+  for (int xb = 0; xb < Nx; xb += XM * nt) {
+    const int x0 = xb + tid;
+    for (int f = 0; f < n; ++f) {
+      double acc[XM];
+      double ev[XM][6];
+      if (err) {  // error inputs for the XM points, loaded before the dictionary
+#pragma unroll
+        for (int i = 0; i < XM; ++i) {
+          const int x = x0 + nt * i < Nx ? x0 + nt * i : Nx - 1;
+          const long idx = f * N + r * Nx + x;
+          ev[i][0] = k1[idx];
+          ev[i][1] = b.k[2][idx];
+          ev[i][2] = b.k[3][idx];
+          ev[i][3] = b.k[4][idx];
+          ev[i][4] = b.k[5][idx];
+          ev[i][5] = yv[idx];
+        }
+      }
+      DE::template eval<XM>(prog, f, vals, Nx, x0, nt, acc);
+#pragma unroll
+      for (int i = 0; i < XM; ++i) {
+        const int x = x0 + nt * i;
+        if (x >= Nx) continue;
+        const long idx = f * N + r * Nx + x;
+        out[idx] = acc[i];
+        fv[f * Nx + x] = acc[i];
+        if (err) {
+          double e = ec.e[0] * ev[i][0];
+          e = __builtin_fma(ec.e[2], ev[i][1], e);
+          e = __builtin_fma(ec.e[3], ev[i][2], e);
+          e = __builtin_fma(ec.e[4], ev[i][3], e);
+          e = __builtin_fma(ec.e[5], ev[i][4], e);
+          e = __builtin_fma(ec.e[6], acc[i], e);
+          const double wgt =
+              __builtin_fma(rtol, fmax(fabs(ev[i][5]), fabs(vals[f * Nx + x])), atol);
+          const double qv = (h * e) / wgt;
+          sq[f * Nx + x] = qv * qv;
+        }
+      }
+    }
+  }
+  if (err) {
+    __syncthreads();
+    for (int s = 1; s < Nx; s <<= 1) {
+      const int per = (Nx + 2 * s - 1) / (2 * s);
+      for (int w = tid; w < n * per; w += nt) {
+        const int f = w / per, i = w - f * per;
+        const int x = i * 2 * s;
+        if (x + s < Nx) sq[f * Nx + x] = sq[f * Nx + x] + sq[f * Nx + x + s];
+      }
+      __syncthreads();
+    }
+    if (tid < n) b.part[(long)tid * g.rows + r] = sq[tid * Nx];
+  }
+  if (nx.ns == 0) return;
+
+  // fused next stage for this row: ytemp = y + h*sum a_j k_j (k_s from LDS)
+  __syncthreads();
+  const bool act = gl < n;
+  const int f = act ? gl : 0;
+  cx v[E];
+  if (act) {
+    const double hs = st->h;
+    const double *y = b.y[st->iy];
+    double *yt = b.y[st->iyt];
+    const double *ks[6];
+    ks[0] = b.k[st->k1s];
+    for (int j = 1; j < 6; ++j) ks[j] = b.k[j];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int j2 = 2 * (t + T * q);
+      const long off = f * N + r * Nx + j2;
+      const double2 yv2 = *reinterpret_cast<const double2 *>(y + off);
+      const double2 a1 = *reinterpret_cast<const double2 *>(ks[0] + off);
+      double ar = nx.a[0] * a1.x, ai = nx.a[0] * a1.y;
+      for (int s = 1; s < nx.ns; ++s) {
+        if (nx.a[s] == 0.0) continue;
+        double kx_, ky_;
+        if (s == nx.ns - 1) {  // the stage just computed
+          kx_ = fv[f * Nx + j2];
+          ky_ = fv[f * Nx + j2 + 1];
+        } else {
+          const double2 kk = *reinterpret_cast<const double2 *>(ks[s] + off);
+          kx_ = kk.x;
+          ky_ = kk.y;
+        }
+        ar = __builtin_fma(nx.a[s], kx_, ar);
+        ai = __builtin_fma(nx.a[s], ky_, ai);
+      }
+      double2 o;
+      o.x = __builtin_fma(hs, ar, yv2.x);
+      o.y = __builtin_fma(hs, ai, yv2.y);
+      *reinterpret_cast<double2 *>(yt + off) = o;
+      v[q] = cx{o.x, o.y};
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cx{0.0, 0.0};
+  }
+  reg_fft<E, -1>(v, lb, 1, tw, px, t, T);
+#pragma unroll
+  for (int q = 0; q < E; ++q) lb[t + T * q] = v[q];
+  __syncthreads();
+  if (!act) return;
+  cx *S = b.S + ((long)f * g.rows + r) * Nxh;
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int k = t + T * q;
+    const cx Av = v[q];
+    const cx Bv = cconj(lb[k == 0 ? 0 : L - k]);
+    const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
+    const cx tt = cmul(D, twN[k]);
+    S[k] = cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)};
+  }
+  if (t == 0) {
+    const cx Av = lb[0];
+    const cx Bv = cconj(lb[0]);
+    const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
+    const cx tt = cmul(D, twN[L]);
+    S[L] = cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)};
+  }
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void k_rowi_reg(Grid g, FftPlan px, const cx *__restrict__ twN,
+                                                  DevState *st, int check_active, Bufs b,
+                                                  FacList fl, DictP dict, int src_kind,
+                                                  const double *src_ptr, int dst_kind, int dst_slot,
+                                                  double *dst_ptr, int err, ErrCoef ec,
+                                                  StageCoef nx, int G) {
+  rowi_body<E, ProgDict>(g, px, twN, st, check_active, b, fl, dict, src_kind, src_ptr, dst_kind,
+                         dst_slot, dst_ptr, err, ec, nx, G);
+}
+
+}  // namespace rd

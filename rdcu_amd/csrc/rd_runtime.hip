@@ -12,17 +12,24 @@
 // attempt, dp45.cu:118).
 
 #include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
 
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "rd_kernels.h"
+#include "rd_kernels_reg.h"
 #include "rdcu_dp45.h"
+
+#include "rd_jit_src.inc"
 
 using rd::cx;
 
@@ -115,6 +122,15 @@ struct rdcu_ctx {
   rd::cx *twN = nullptr;
   rd::Bufs bufs{};
   rd::Dict dict{};
+  rd::DictP dictp{};
+  bool dictk_ok = false;
+  // x-inverse kernel with the dictionary compiled in (hiprtc); null if not built
+  hipModule_t jit_mod = nullptr;
+  hipFunction_t jit_rowi = nullptr;
+  std::string jit_log;
+  std::vector<int> h_tfield, h_tstart, h_fslot, h_fpow;
+  std::vector<double> h_tcoef;
+  int col_lds = 0;
   rd::FacList fl_used{}, fl_all{};
   rd::DevState *st = nullptr;
   rd::DevState *sh = nullptr;  // pinned host shadow
@@ -128,6 +144,12 @@ struct rdcu_ctx {
   int c2r_grp = 1, c2r_grp_all = 1, c2r_thr = 64;
   size_t lds_r2c = 0, lds_col = 0, lds_c2r = 0, lds_c2r_all = 0;
   long nlines = 0, nred = 0;
+  // register-resident path: elements per thread for the x axis and the
+  // strided axes (0 = not eligible); shapes picked by the autotuner
+  int reg = 0, Ex = 0, E0 = 0, E1 = 0;
+  int rf_rpb = 1, cr_tc = 1, ri_G = 1, cr_fpw = 1, max_fpf = 1;
+  size_t lds_rf = 0, lds_cr0 = 0, lds_cr1 = 0, lds_ri = 0;
+  std::string tune_log;
   // host-side flags and counters
   int fixed = 0;
   bool k1_valid = false;
@@ -135,6 +157,7 @@ struct rdcu_ctx {
 
   ~rdcu_ctx() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (jit_mod) (void)hipModuleUnload(jit_mod);
     if (graph) (void)hipGraphDestroy(graph);
     for (void *p : allocs) (void)hipFree(p);
     if (sh) (void)hipHostFree(sh);
@@ -178,10 +201,17 @@ struct rdcu_ctx {
 
 namespace {
 
-// Enqueues one RHS evaluation.  stage 2..7: input = DP45 stage sum, output =
-// stage slot; stage 0: input/output given by kind/pointer.
-int enqueue_rhs(rdcu_ctx *c, int stage, int src_kind, const double *src_ptr, int dst_kind,
-                int dst_slot, double *dst_ptr, int check, int err) {
+// Kernel launches of one RHS evaluation, in order.  stage 2..7: input =
+// DP45 stage sum, output = stage slot; stage 0: input/output given by
+// kind/pointer.
+struct Launch {
+  const char *name;
+  std::function<void()> go;
+};
+
+std::vector<Launch> rhs_launches(rdcu_ctx *c, int stage, int src_kind, const double *src_ptr,
+                                 int dst_kind, int dst_slot, double *dst_ptr, int check, int err) {
+  std::vector<Launch> v;
   rd::StageCoef sc{};
   int rsrc = src_kind;
   if (stage >= 2) {
@@ -191,16 +221,21 @@ int enqueue_rhs(rdcu_ctx *c, int stage, int src_kind, const double *src_ptr, int
   }
   const long nrows = (long)c->n * c->rows;
   dim3 g1((unsigned)((nrows + c->r2c_rpb - 1) / c->r2c_rpb));
-  hipLaunchKernelGGL(rd::k_row_r2c, g1, dim3(c->r2c_thr), c->lds_r2c, c->stream, c->grid, c->px,
-                     c->twN, c->st, check, c->bufs, rsrc, src_ptr, sc, c->r2c_rpb);
+  v.push_back({"row_r2c", [=]() {
+                 hipLaunchKernelGGL(rd::k_row_r2c, g1, dim3(c->r2c_thr), c->lds_r2c, c->stream,
+                                    c->grid, c->px, c->twN, c->st, check, c->bufs, rsrc, src_ptr,
+                                    sc, c->r2c_rpb);
+               }});
   const int nd = c->ndim;
   const int ntc = (c->Nxh + c->col_tc - 1) / c->col_tc;
   if (nd == 3) {
     rd::ColGeom cg{c->Ns[1], (long)c->Nxh, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxh, ntc,
                    c->col_tc, 1};
     dim3 gg((unsigned)((long)c->n * c->Ns[0] * ntc));
-    hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream, c->grid, c->p1, cg,
-                       c->st, check, c->bufs, c->fl_used, 0);
+    v.push_back({"col_fwd", [=]() {
+                   hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream,
+                                      c->grid, c->p1, cg, c->st, check, c->bufs, c->fl_used, 0);
+                 }});
   }
   if (nd >= 2) {
     rd::ColGeom cg{};
@@ -219,25 +254,435 @@ int enqueue_rhs(rdcu_ctx *c, int stage, int src_kind, const double *src_ptr, int
       cg.gs2 = c->Nxh;
     }
     dim3 gg((unsigned)((long)c->n * cg.gdiv * ntc));
-    hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream, c->grid, c->p0, cg,
-                       c->st, check, c->bufs, c->fl_used, 1);
+    v.push_back({"col_mult", [=]() {
+                   hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream,
+                                      c->grid, c->p0, cg, c->st, check, c->bufs, c->fl_used, 1);
+                 }});
   }
   if (nd == 3 && c->nf_used > 0) {
     rd::ColGeom cg{c->Ns[1], (long)c->Nxh, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxh, ntc,
                    c->col_tc, 1};
     dim3 gg((unsigned)((long)c->nf_used * c->Ns[0] * ntc));
-    hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream, c->grid, c->p1, cg,
-                       c->st, check, c->bufs, c->fl_used, 2);
+    v.push_back({"col_inv", [=]() {
+                   hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream,
+                                      c->grid, c->p1, cg, c->st, check, c->bufs, c->fl_used, 2);
+                 }});
   }
   rd::ErrCoef ec{};
   for (int j = 0; j < 7; ++j) ec.e[j] = EE[j];
   int csrc = stage >= 2 ? rd::SRC_YT : src_kind;
-  hipLaunchKernelGGL(rd::k_row_c2r, dim3((unsigned)c->rows), dim3(c->c2r_thr), c->lds_c2r,
-                     c->stream, c->grid, c->px, c->twN, c->st, check, c->bufs, c->fl_used, c->dict,
-                     0, csrc, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, c->c2r_grp,
-                     (double *)nullptr);
+  v.push_back({"row_c2r", [=]() {
+                 hipLaunchKernelGGL(rd::k_row_c2r, dim3((unsigned)c->rows), dim3(c->c2r_thr),
+                                    c->lds_c2r, c->stream, c->grid, c->px, c->twN, c->st, check,
+                                    c->bufs, c->fl_used, c->dict, 0, csrc, src_ptr, dst_kind,
+                                    dst_slot, dst_ptr, err, ec, c->c2r_grp, (double *)nullptr);
+               }});
+  return v;
+}
+
+std::vector<Launch> rhs_launches_reg(rdcu_ctx *c, int src_kind, const double *src_ptr, int dst_kind,
+                                     int dst_slot, double *dst_ptr, int check);
+
+int enqueue_rhs(rdcu_ctx *c, int stage, int src_kind, const double *src_ptr, int dst_kind,
+                int dst_slot, double *dst_ptr, int check, int err) {
+  if (c->reg && stage == 0 && !err) {
+    for (auto &l : rhs_launches_reg(c, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, check))
+      l.go();
+    HIPCHECK(hipGetLastError());
+    c->nrhs++;
+    return RDCU_OK;
+  }
+  for (auto &l : rhs_launches(c, stage, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, check, err))
+    l.go();
   HIPCHECK(hipGetLastError());
   c->nrhs++;
+  return RDCU_OK;
+}
+
+// ---- register-resident path ------------------------------------------------
+
+// Elements per thread of the register Stockham for a length (0 = use the
+// LDS path): 8 for powers of two >= 8, 24 for 3*2^k >= 24.
+int reg_elems(int L) {
+  auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+  if (L >= 8 && pow2(L)) return 8;
+  if (L >= 24 && L % 3 == 0 && pow2(L / 3)) return 24;
+  return 0;
+}
+
+template <int E>
+void launch_rowf(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, rd::StageCoef sc,
+                 int rpb) {
+  const long nrows = (long)c->n * c->rows;
+  const int T = c->L / E;
+  dim3 g((unsigned)((nrows + rpb - 1) / rpb));
+  size_t lds = (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx);
+  hipLaunchKernelGGL(rd::k_rowf_reg<E>, g, dim3(rpb * T), lds, c->stream, c->grid, c->px, c->twN,
+                     c->st, check, c->bufs, src_kind, src_ptr, sc, rpb);
+}
+
+template <int E>
+void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups, int check,
+                 const rd::FacList &fl, int mode) {
+  const int T = cg.L / E;
+  const int fpw = std::max(1, c->cr_fpw);
+  cg.nparts = mode == 1 ? (c->max_fpf + fpw - 1) / fpw : 1;
+  if (cg.nparts < 1) cg.nparts = 1;
+  dim3 g((unsigned)(ngroups * cg.ntc * cg.nparts));
+  size_t lds = (size_t)(cg.L + cg.tc * rd::padded_len(cg.L) + (mode == 1 ? cg.tc * cg.L : 0)) *
+                   sizeof(cx) +
+               (mode == 1 ? (size_t)cg.L * sizeof(double) : 0);
+  hipLaunchKernelGGL(rd::k_col_reg<E>, g, dim3(cg.tc * T), lds, c->stream, c->grid, p, cg, c->st,
+                     check, c->bufs, fl, mode, fpw);
+}
+
+template <int E>
+void launch_rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, int dst_kind,
+                 int dst_slot, double *dst_ptr, int err, rd::StageCoef nx, int G) {
+  const int T = c->L / E;
+  rd::ErrCoef ec{};
+  for (int j = 0; j < 7; ++j) ec.e[j] = EE[j];
+  size_t lds = (size_t)(c->L + G * rd::padded_len(c->L)) * sizeof(cx) +
+               (size_t)(2 * c->n + c->nf_used) * c->Nx * sizeof(double) +
+               (size_t)c->dictp.nwords * sizeof(int);
+  if (c->jit_rowi) {
+    rd::Grid a_g = c->grid;
+    rd::FftPlan a_px = c->px;
+    const rd::cx *a_tw = c->twN;
+    rd::DevState *a_st = c->st;
+    int a_check = check;
+    rd::Bufs a_b = c->bufs;
+    rd::FacList a_fl = c->fl_used;
+    rd::DictP a_d = c->dictp;
+    int a_src = src_kind;
+    const double *a_sp = src_ptr;
+    int a_dk = dst_kind, a_ds = dst_slot;
+    double *a_dp = dst_ptr;
+    int a_err = err;
+    rd::ErrCoef a_ec = ec;
+    rd::StageCoef a_nx = nx;
+    int a_G = G;
+    void *args[] = {&a_g, &a_px, &a_tw, &a_st, &a_check, &a_b, &a_fl, &a_d, &a_src,
+                    &a_sp, &a_dk, &a_ds, &a_dp, &a_err, &a_ec, &a_nx, &a_G};
+    (void)hipModuleLaunchKernel(c->jit_rowi, (unsigned)c->rows, 1, 1, (unsigned)(G * T), 1, 1,
+                                (unsigned)lds, c->stream, args, nullptr);
+    return;
+  }
+  hipLaunchKernelGGL(rd::k_rowi_reg<E>, dim3((unsigned)c->rows), dim3(G * T), lds, c->stream,
+                     c->grid, c->px, c->twN, c->st, check, c->bufs, c->fl_used, c->dictp, src_kind,
+                     src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G);
+}
+
+void rowf(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, rd::StageCoef sc, int rpb) {
+  if (c->Ex == 8) launch_rowf<8>(c, check, src_kind, src_ptr, sc, rpb);
+  else launch_rowf<24>(c, check, src_kind, src_ptr, sc, rpb);
+}
+void colr(rdcu_ctx *c, const rd::FftPlan &p, int E, rd::ColGeom cg, long ngroups, int check,
+          const rd::FacList &fl, int mode) {
+  if (E == 8) launch_colr<8>(c, p, cg, ngroups, check, fl, mode);
+  else launch_colr<24>(c, p, cg, ngroups, check, fl, mode);
+}
+void rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, int dst_kind, int dst_slot,
+          double *dst_ptr, int err, rd::StageCoef nx, int G) {
+  if (c->Ex == 8) launch_rowi<8>(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, nx, G);
+  else launch_rowi<24>(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, nx, G);
+}
+
+rd::StageCoef stage_coef(int stage) {
+  rd::StageCoef sc{};
+  if (stage >= 2) {
+    sc.ns = stage - 1;
+    for (int j = 0; j < sc.ns; ++j) sc.a[j] = STAGE_A[stage - 2][j];
+  }
+  return sc;
+}
+
+// Strided passes of the register path for one RHS (after the x forward pass).
+std::vector<Launch> col_launches_reg(rdcu_ctx *c, int check, int tc) {
+  std::vector<Launch> v;
+  if (c->col_lds) {  // LDS column kernels inside the register path (comparison switch)
+    for (auto &l : rhs_launches(c, 0, rd::SRC_Y, nullptr, rd::DST_PTR, 0, nullptr, check, 0))
+      if (std::string(l.name).rfind("col", 0) == 0) v.push_back(l);
+    return v;
+  }
+  const int nd = c->ndim;
+  const int ntc = (c->Nxh + tc - 1) / tc;
+  if (nd == 3) {
+    rd::ColGeom cg{c->Ns[1], (long)c->Nxh, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxh, ntc, tc, 1};
+    long ng = (long)c->n * c->Ns[0];
+    v.push_back({"col_fwd", [=]() { colr(c, c->p1, c->E1, cg, ng, check, c->fl_used, 0); }});
+  }
+  if (nd >= 2) {
+    rd::ColGeom cg{};
+    cg.L = c->Ns[0];
+    cg.ntc = ntc;
+    cg.tc = tc;
+    cg.gs1 = c->Nh;
+    if (nd == 2) {
+      cg.es = c->Nxh;
+      cg.gdiv = 1;
+    } else {
+      cg.es = (long)c->Ns[1] * c->Nxh;
+      cg.gdiv = c->Ns[1];
+      cg.gs2 = c->Nxh;
+    }
+    long ng = (long)c->n * cg.gdiv;
+    v.push_back({"col_mult", [=]() { colr(c, c->p0, c->E0, cg, ng, check, c->fl_used, 1); }});
+  }
+  if (nd == 3 && c->nf_used > 0) {
+    rd::ColGeom cg{c->Ns[1], (long)c->Nxh, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxh, ntc, tc, 1};
+    long ng = (long)c->nf_used * c->Ns[0];
+    v.push_back({"col_inv", [=]() { colr(c, c->p1, c->E1, cg, ng, check, c->fl_used, 2); }});
+  }
+  return v;
+}
+
+// One plain RHS on the register path (no stage fusion).
+std::vector<Launch> rhs_launches_reg(rdcu_ctx *c, int src_kind, const double *src_ptr, int dst_kind,
+                                     int dst_slot, double *dst_ptr, int check) {
+  std::vector<Launch> v;
+  rd::StageCoef none{};
+  v.push_back({"row_fwd", [=]() { rowf(c, check, src_kind, src_ptr, none, c->rf_rpb); }});
+  for (auto &l : col_launches_reg(c, check, c->cr_tc)) v.push_back(l);
+  v.push_back({"row_inv", [=]() {
+                 rowi(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, 0, none, c->ri_G);
+               }});
+  return v;
+}
+
+// One DP45 attempt on the register path: x forward of stage 2, then per
+// stage the strided passes and the x inverse pass fused with the next
+// stage's sum and x forward pass; stage 7 adds the error tree.
+std::vector<Launch> attempt_launches_reg(rdcu_ctx *c) {
+  std::vector<Launch> v;
+  rd::StageCoef s2 = stage_coef(2);
+  v.push_back({"row_fwd", [=]() { rowf(c, 1, rd::SRC_STAGE, nullptr, s2, c->rf_rpb); }});
+  for (int s = 2; s <= 7; ++s) {
+    for (auto &l : col_launches_reg(c, 1, c->cr_tc)) v.push_back(l);
+    rd::StageCoef nx = s < 7 ? stage_coef(s + 1) : rd::StageCoef{};
+    int dk = s < 7 ? rd::DST_KS : rd::DST_K7;
+    int err = s == 7 && !c->fixed;
+    v.push_back({"row_inv", [=]() {
+                   rowi(c, 1, rd::SRC_YT, nullptr, dk, s - 1, nullptr, err, nx, c->ri_G);
+                 }});
+  }
+  return v;
+}
+
+// Straight-line evaluator of this problem's dictionary: per field, terms in
+// input order, term = 1 * v^p * ..., acc = fma(C, term, acc) from 0 -- the
+// same operations as dict_field, with slots, powers and coefficients as
+// compile-time constants.
+std::string jit_dict_source(rdcu_ctx *c) {
+  auto powexpr = [](const std::string &v, int p) {
+    if (p == 0) return std::string("1.0");
+    int a = p < 0 ? -p : p;
+    std::string e = v;
+    for (int j = 1; j < a; ++j) e = "(" + e + " * " + v + ")";
+    return p < 0 ? "(1.0 / " + e + ")" : e;
+  };
+  std::string s;
+  s += "namespace rd {\nstruct JitDict {\n  static constexpr bool kProg = false;\n";
+  s += "  template <int XM>\n  __device__ __forceinline__ static void eval(const int *, int f, "
+       "const double *vals, int Nx, int x0, int stride, double (&acc)[XM]) {\n";
+  s += "#pragma unroll\n    for (int i = 0; i < XM; ++i) {\n";
+  s += "      const int x = x0 + stride * i;\n      const bool in = x < Nx;\n";
+  s += "      double a = 0.0;\n      switch (f) {\n";
+  char buf[256];
+  const int nt = (int)c->h_tfield.size();
+  for (int f = 0; f < c->n; ++f) {
+    snprintf(buf, sizeof buf, "      case %d: {\n", f);
+    s += buf;
+    std::vector<char> used(c->n + c->nf_used, 0);
+    for (int i = 0; i < nt; ++i)
+      if (c->h_tfield[i] == f)
+        for (int j = c->h_tstart[i]; j < c->h_tstart[i + 1]; ++j) used[c->h_fslot[j]] = 1;
+    for (size_t sl = 0; sl < used.size(); ++sl)
+      if (used[sl]) {
+        snprintf(buf, sizeof buf, "        const double v%zu = in ? vals[%zu * Nx + x] : 0.0;\n", sl,
+                 sl);
+        s += buf;
+      }
+    for (int i = 0; i < nt; ++i) {
+      if (c->h_tfield[i] != f) continue;
+      s += "        { double t = 1.0;";
+      for (int j = c->h_tstart[i]; j < c->h_tstart[i + 1]; ++j)
+        s += " t = t * " + powexpr("v" + std::to_string(c->h_fslot[j]), c->h_fpow[j]) + ";";
+      snprintf(buf, sizeof buf, " a = __builtin_fma(%a, t, a); }\n", c->h_tcoef[i]);
+      s += buf;
+    }
+    s += "      } break;\n";
+  }
+  s += "      default: break;\n      }\n      acc[i] = a;\n    }\n  }\n};\n}  // namespace rd\n";
+  return s;
+}
+
+// Compiles the x-inverse kernel with the dictionary built in.  On any failure
+// the interpreter kernel (k_rowi_reg) stays in use; RDCU_JIT=0 disables it.
+void build_jit(rdcu_ctx *c) {
+  const char *env = getenv("RDCU_JIT");
+  if (env && !strcmp(env, "0")) return;
+  std::string src = "#include \"rd_kernels_reg.h\"\n" + jit_dict_source(c);
+  src += "extern \"C\" __global__ __launch_bounds__(256) void rdcu_rowi_jit(rd::Grid g, rd::FftPlan px, "
+         "const rd::cx *__restrict__ twN, rd::DevState *st, int check_active, rd::Bufs b, "
+         "rd::FacList fl, rd::DictP dict, int src_kind, const double *src_ptr, int dst_kind, "
+         "int dst_slot, double *dst_ptr, int err, rd::ErrCoef ec, rd::StageCoef nx, int G) {\n";
+  src += "  rd::rowi_body<" + std::to_string(c->Ex) +
+         ", rd::JitDict>(g, px, twN, st, check_active, b, fl, dict, src_kind, src_ptr, dst_kind, "
+         "dst_slot, dst_ptr, err, ec, nx, G);\n}\n";
+  hipDeviceProp_t prop;
+  std::string arch = "--offload-arch=gfx950";
+  if (hipGetDeviceProperties(&prop, c->dev) == hipSuccess) {
+    std::string a = prop.gcnArchName;
+    arch = "--offload-arch=" + a.substr(0, a.find(':'));
+  }
+  // code objects already compiled in this process, keyed by source and target
+  static std::mutex cache_mu;
+  static std::map<std::string, std::vector<char>> cache;
+  const std::string key = arch + "\n" + src;
+  std::vector<char> code;
+  {
+    std::lock_guard<std::mutex> lk(cache_mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) code = it->second;
+  }
+  if (code.empty()) {
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "rdcu_rowi_jit.hip", kJitNumHeaders, kJitHeaders,
+                          kJitHeaderNames) != HIPRTC_SUCCESS) {
+    c->jit_log = "hiprtcCreateProgram failed";
+    return;
+  }
+  const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  hiprtcResult r = hiprtcCompileProgram(prog, 9, opts);
+  if (r != HIPRTC_SUCCESS) {
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string log(ls, '\0');
+    if (ls) hiprtcGetProgramLog(prog, &log[0]);
+    c->jit_log = "hiprtc: " + log;
+    hiprtcDestroyProgram(&prog);
+    return;
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  code.resize(cs);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  std::lock_guard<std::mutex> lk(cache_mu);
+  cache[key] = code;
+  }
+  if (hipModuleLoadData(&c->jit_mod, code.data()) != hipSuccess ||
+      hipModuleGetFunction(&c->jit_rowi, c->jit_mod, "rdcu_rowi_jit") != hipSuccess) {
+    c->jit_log = "module load failed";
+    if (c->jit_mod) (void)hipModuleUnload(c->jit_mod);
+    c->jit_mod = nullptr;
+    c->jit_rowi = nullptr;
+    return;
+  }
+  c->jit_log = "ok";
+}
+
+// Autotuner: every candidate shape of each kernel computes the same values
+// (the arithmetic does not depend on the tiling), so the fastest is kept.
+// Candidates are timed alone with HIP events on the plain RHS of the
+// current state.
+int autotune(rdcu_ctx *c) {
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  auto time_of = [&](const std::function<void()> &f) -> double {
+    for (int i = 0; i < 2; ++i) f();
+    (void)hipEventRecord(e0, c->stream);
+    const int reps = 10;
+    for (int i = 0; i < reps; ++i) f();
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (hipGetLastError() != hipSuccess) return 1e30;
+    return ms / reps;
+  };
+  const int T = c->L / c->Ex;
+  rd::StageCoef none{};
+  char line[160];
+  // x forward: rows per block
+  double best = 1e30;
+  for (int rpb = 1; rpb <= 16; rpb *= 2) {
+    if (rpb * T > 256 || (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx) > 160 * 1024) break;
+    double tm = time_of([&]() { rowf(c, 0, rd::SRC_Y, nullptr, none, rpb); });
+    snprintf(line, sizeof line, "row_fwd rpb=%d %.2fus; ", rpb, tm * 1e3);
+    c->tune_log += line;
+    if (tm < best) {
+      best = tm;
+      c->rf_rpb = rpb;
+    }
+  }
+  // strided passes: column tile width
+  if (c->ndim >= 2) {
+    best = 1e30;
+    int Lmax = c->Ns[0];
+    int Emin = c->E0;
+    if (c->ndim == 3) {
+      Lmax = std::max(Lmax, c->Ns[1]);
+      Emin = std::min(c->E0, c->E1);
+    }
+    int best_fpw = c->max_fpf;
+    for (int tc = 1; tc <= 16; tc *= 2) {
+      if (tc * (Lmax / Emin) > 256 ||
+          (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) > 160 * 1024)
+        break;
+      for (int fpw = c->max_fpf; fpw >= 1; fpw = fpw > 1 ? (fpw + 1) / 2 : 0) {
+        c->cr_fpw = fpw;
+        auto ls = col_launches_reg(c, 0, tc);
+        double tm = time_of([&]() { for (auto &l : ls) l.go(); });
+        snprintf(line, sizeof line, "col tc=%d fpw=%d %.2fus; ", tc, fpw, tm * 1e3);
+        c->tune_log += line;
+        if (tm < best) {
+          best = tm;
+          c->cr_tc = tc;
+          best_fpw = fpw;
+        }
+        if (fpw == 1) break;
+      }
+    }
+    c->cr_fpw = best_fpw;
+  }
+  // x inverse: factor lines per group
+  best = 1e30;
+  for (int G = c->n; G <= std::max(c->n, c->nf_used); ++G) {
+    if (G * T > 256) break;
+    size_t lds = (size_t)(c->L + G * rd::padded_len(c->L)) * sizeof(cx) +
+                 (size_t)(2 * c->n + c->nf_used) * c->Nx * 8 + (size_t)c->dictp.nwords * 4;
+    if (lds > 160 * 1024) break;
+    double tm = time_of([&]() {
+      rowi(c, 0, rd::SRC_Y, nullptr, rd::DST_PTR, 0, c->rhs_out, 0, none, G);
+    });
+    snprintf(line, sizeof line, "row_inv G=%d %.2fus; ", G, tm * 1e3);
+    c->tune_log += line;
+    if (tm < best) {
+      best = tm;
+      c->ri_G = G;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipGetLastError());
+  return RDCU_OK;
+}
+
+// Raise the dynamic-LDS limit of every register-path kernel to the 160 KB of
+// a gfx950 CU.
+int allow_big_lds() {
+  const int lim = 160 * 1024;
+  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowf_reg<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowf_reg<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_col_reg<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_col_reg<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_reg<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_reg<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return RDCU_OK;
 }
 
@@ -281,8 +726,13 @@ int enqueue_all_factors(rdcu_ctx *c) {
   return RDCU_OK;
 }
 
+std::vector<Launch> attempt_launches_reg(rdcu_ctx *c);
+
 int enqueue_attempt(rdcu_ctx *c) {
-  for (int s = 2; s <= 7; ++s) {
+  if (c->reg) {
+    for (auto &l : attempt_launches_reg(c)) l.go();
+  }
+  for (int s = 2; !c->reg && s <= 7; ++s) {
     int dk = s < 7 ? rd::DST_KS : rd::DST_K7;
     int st = enqueue_rhs(c, s, rd::SRC_STAGE, nullptr, dk, s - 1, nullptr, 1, s == 7 && !c->fixed);
     if (st) return st;
@@ -436,6 +886,9 @@ int rdcu_create(const rdcu_problem *p, double atol, double rtol, int fixedstep, 
   astart[c->n] = (int)ablk.size();
   for (int f = 0; f < c->n; ++f) slot_of[f] = f;
   c->nf_used = (int)ublk.size();
+  c->max_fpf = 1;
+  for (int f = 0; f < c->n; ++f) c->max_fpf = std::max(c->max_fpf, ustart[f + 1] - ustart[f]);
+  c->cr_fpw = c->max_fpf;
   c->nf_all = (int)ablk.size();
   std::vector<int> tfield, tstart, fslot, fpow;
   std::vector<double> tcoef;
@@ -499,6 +952,37 @@ int rdcu_create(const rdcu_problem *p, double atol, double rtol, int fixedstep, 
         (st = c->upload(&d_fp, fpow)))
       return st;
     c->dict = rd::Dict{p->nterms, d_tf, d_tc, d_ts, d_fs, d_fp};
+    // the same dictionary as a program for the register path: per-field term
+    // lists (input order kept), records nfac, coef (2 words), (slot, pow)...
+    std::vector<int> prog(c->n + 1, 0);
+    for (int f = 0; f < c->n; ++f) {
+      prog[f] = (int)prog.size();
+      for (int i = 0; i < p->nterms; ++i) {
+        if (tfield[i] != f) continue;
+        const int nfac_i = tstart[i + 1] - tstart[i];
+        int lo, hi;
+        std::memcpy(&lo, reinterpret_cast<const char *>(&tcoef[i]), 4);
+        std::memcpy(&hi, reinterpret_cast<const char *>(&tcoef[i]) + 4, 4);
+        prog.push_back(nfac_i);
+        prog.push_back(lo);
+        prog.push_back(hi);
+        for (int j = tstart[i]; j < tstart[i + 1]; ++j) {
+          prog.push_back(fslot[j]);
+          prog.push_back(fpow[j]);
+        }
+      }
+    }
+    prog[c->n] = (int)prog.size();
+    bool ok = prog.size() <= 4096;
+    int *d_prog = nullptr;
+    if ((st = c->upload(&d_prog, prog))) return st;
+    c->dictp = rd::DictP{d_prog, (int)prog.size()};
+    c->h_tfield = tfield;
+    c->h_tstart = tstart;
+    c->h_fslot = fslot;
+    c->h_fpow = fpow;
+    c->h_tcoef = tcoef;
+    c->dictk_ok = ok;
     int *ub, *uf, *us, *ab, *af, *as;
     if ((st = c->upload(&ub, ublk)) || (st = c->upload(&uf, ufld)) || (st = c->upload(&us, ustart)) ||
         (st = c->upload(&ab, ablk)) || (st = c->upload(&af, afld)) || (st = c->upload(&as, astart)))
@@ -507,6 +991,29 @@ int rdcu_create(const rdcu_problem *p, double atol, double rtol, int fixedstep, 
     c->fl_all = rd::FacList{c->nf_all, ab, af, as};
   }
   if ((st = choose_shapes(c.get()))) return st;
+  // register-resident path when every transform length allows it
+  c->Ex = reg_elems(c->L);
+  c->E0 = c->ndim >= 2 ? reg_elems(c->Ns[0]) : 0;
+  c->E1 = c->ndim == 3 ? reg_elems(c->Ns[1]) : 0;
+  {
+    bool ok = c->Ex > 0 && (c->ndim < 2 || c->E0 > 0) && (c->ndim < 3 || c->E1 > 0);
+    const int T = c->Ex ? c->L / c->Ex : 0;
+    ok = ok && c->n * T <= 256;  // one block of <= 256 threads per row position
+    ok = ok && (size_t)(c->L + c->n * rd::padded_len(c->L)) * sizeof(cx) +
+                       (size_t)(2 * c->n + c->nf_used) * c->Nx * 8 + (size_t)c->dictp.nwords * 4 <=
+                   160 * 1024;
+    ok = ok && c->dictk_ok;
+    if (c->ndim >= 2) ok = ok && c->Ns[0] / c->E0 <= 256 && (size_t)3 * c->Ns[0] * sizeof(cx) <= 150 * 1024;
+    if (c->ndim == 3) ok = ok && c->Ns[1] / c->E1 <= 256 && (size_t)2 * c->Ns[1] * sizeof(cx) <= 160 * 1024;
+    c->reg = ok ? 1 : 0;
+    // RDCU_PATH=lds selects the LDS-resident kernels for every size (the two
+    // paths compute identical values; this exists to compare them)
+    const char *path = getenv("RDCU_PATH");
+    if (path && !strcmp(path, "lds")) c->reg = 0;
+    const char *colp = getenv("RDCU_COL");
+    if (colp && !strcmp(colp, "lds")) c->col_lds = 1;
+    c->ri_G = c->n;
+  }
 
   // buffers
   const size_t nN = (size_t)c->n * c->N;
@@ -547,6 +1054,19 @@ int rdcu_create(const rdcu_problem *p, double atol, double rtol, int fixedstep, 
     c->sh->halted = 1;
     if ((st = c->dalloc(&c->st, 1))) return st;
     HIPCHECK(hipMemcpy(c->st, c->sh, sizeof(rd::DevState), hipMemcpyHostToDevice));
+  }
+
+  if (c->reg) {
+    static bool lds_raised = false;
+    if (!lds_raised) {
+      if ((st = allow_big_lds())) return st;
+      lds_raised = true;
+    }
+    build_jit(c.get());
+    if ((st = autotune(c.get()))) return st;
+    const char *vb = getenv("RDCU_VERBOSE");
+    if (vb && *vb && *vb != '0')
+      fprintf(stderr, "rdcu jit: %s\nrdcu autotune: %s\n", c->jit_log.c_str(), c->tune_log.c_str());
   }
 
   // capture one attempt
@@ -625,6 +1145,70 @@ int rdcu_run(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double
   }
   *t = c->sh->t;
   *h = c->sh->h;
+  return RDCU_OK;
+}
+
+int rdcu_advance(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double t1,
+                 long long max_attempts, long long *done) {
+  if (!c || !t || !h) return fail(RDCU_EINVAL, "null argument");
+  HIPCHECK(hipSetDevice(c->dev));
+  int st;
+  if ((st = begin(c, INFINITY, 1, *t, *h, hmin, hmax, t1))) return st;
+  c->k1_valid = false;
+  if ((st = ensure_k1(c))) return st;
+  long long k = 0;
+  double tc = *t, hc = *h;
+  while (k < max_attempts && tc < t1 && hc > hmin) {
+    long long K = std::min<long long>(kChunkMax, max_attempts - k);
+    if ((st = begin(c, INFINITY, 0, 0, 0, hmin, hmax, t1))) return st;
+    for (long long i = 0; i < K; ++i) HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+    if ((st = read_state(c))) return st;
+    const int nl = c->sh->nlog;
+    c->nrhs += 6L * nl;
+    k += nl;
+    tc = c->sh->t;
+    hc = c->sh->h;
+    if (nl < K) break;
+  }
+  *t = c->sh->t;
+  *h = c->sh->h;
+  if (done) *done = k;
+  return RDCU_OK;
+}
+
+int rdcu_time_rhs(rdcu_ctx *c, int reps, double *ms, int max, int *count, char *names,
+                  int names_len) {
+  if (!c || !ms || max < 1 || reps < 1) return fail(RDCU_EINVAL, "bad argument");
+  HIPCHECK(hipSetDevice(c->dev));
+  auto ls = c->reg ? rhs_launches_reg(c, rd::SRC_Y, nullptr, rd::DST_PTR, 0, c->rhs_out, 0)
+                   : rhs_launches(c, 0, rd::SRC_Y, nullptr, rd::DST_PTR, 0, c->rhs_out, 0, 0);
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  auto timed = [&](const std::function<void()> &body, double *out) -> int {
+    for (int r = 0; r < 3; ++r) body();  // warm
+    HIPCHECK(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) body();
+    HIPCHECK(hipEventRecord(e1, c->stream));
+    HIPCHECK(hipEventSynchronize(e1));
+    float f = 0;
+    HIPCHECK(hipEventElapsedTime(&f, e0, e1));
+    *out = (double)f / reps;
+    return RDCU_OK;
+  };
+  std::string nm = "rhs";
+  int cnt = 0, st = RDCU_OK;
+  st = timed([&]() { for (auto &l : ls) l.go(); }, &ms[cnt++]);
+  for (size_t i = 0; !st && i < ls.size() && cnt < max; ++i) {
+    st = timed(ls[i].go, &ms[cnt++]);
+    nm += std::string(",") + ls[i].name;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  HIPCHECK(hipGetLastError());
+  if (count) *count = cnt;
+  if (names && names_len > 0) snprintf(names, names_len, "%s", nm.c_str());
   return RDCU_OK;
 }
 

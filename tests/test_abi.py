@@ -1,0 +1,54 @@
+"""The C ABI boundary: both libraries load and export every function that
+include/rdcu_dp45.h declares (no GPU call is made here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ORACLE_LIB, ROOT
+
+HEADER = os.path.join(ROOT, "include", "rdcu_dp45.h")
+HIP_LIB = os.path.join(ROOT, "rdcu_amd", "lib", "librdcu_hip.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(rdcu_[a-z_]+)\s*\(", text))
+    names.discard("rdcu_step_fn")
+    return sorted(names)
+
+
+def test_header_declares_the_boundary():
+    names = declared()
+    for n in ("rdcu_create", "rdcu_step", "rdcu_run", "rdcu_eval", "rdcu_destroy", "rdcu_rhs"):
+        assert n in names
+
+
+@pytest.mark.parametrize("path", [ORACLE_LIB, HIP_LIB], ids=["oracle", "hip"])
+def test_library_exports_every_declared_symbol(path):
+    assert os.path.exists(path), f"{path} not built"
+    lib = ctypes.CDLL(path)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_symbol():
+    from rdcu_amd.abi import SYMBOLS
+    assert sorted(SYMBOLS) == declared()
+
+
+def test_backend_names():
+    from rdcu_amd.abi import load
+    assert load(ORACLE_LIB).rdcu_backend() == b"cpu-oracle"
+    assert load(HIP_LIB).rdcu_backend() == b"hip-gfx950"
+
+
+def test_hip_cli_help_runs_without_gpu(tmp_path):
+    """-h prints the help and index listing before any device call."""
+    import subprocess
+    exe = os.path.join(ROOT, "rdcu_amd", "bin", "rdcu")
+    r = subprocess.run([exe, "-n", "2", "-N", "8,8", "-h"], capture_output=True, text=True,
+                       timeout=60, cwd=tmp_path)
+    assert r.returncode == 0 and "13: u1_11" in r.stdout

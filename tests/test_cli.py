@@ -1,0 +1,143 @@
+"""The drop-in CLI (rdcu_amd/csrc/rdcu_cli.cpp), exercised through the oracle
+build rdcu_cpu on the CPU: flags, help listing, coupling file, initial
+conditions, output files, restart.  Golden text comes from the reference
+notebook (tests/golden/notebook_runs.json)."""
+import json
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ORACLE_CLI, ROOT
+from rdcu_amd import cgle_terms, to_cli
+
+GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "notebook_runs.json")))
+C1 = sum((["-c", c] for c in to_cli(cgle_terms(1))), [])
+
+
+def run(args, cwd, exe=ORACLE_CLI, rc=0):
+    r = subprocess.run([exe] + [str(a) for a in args], cwd=cwd, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == rc, r.stdout + r.stderr
+    return r
+
+
+def test_help_index_listing_matches_reference(tmp_path):
+    r = run(["-N", "256,256", "-L", "100,100", "-n", "2", "-h"], tmp_path)
+    lines = r.stdout.rstrip("\n").split("\n")
+    gold = GOLD["index_listing"]
+    assert lines[-len(gold):] == gold
+
+
+def test_verbose_equations_match_reference(tmp_path):
+    """The coupling file of cgle.ipynb cell 3 (b=2, c=0.8, '%f' format) gives
+    the equation lines printed by the reference."""
+    b, c = 2.0, 0.8
+    couplings = ["1.0,0,0,1", "1.0,0,6,1", "1.0,0,12,1", "-%f,0,7,1" % b, "-%f,0,13,1" % b,
+                 "-1.0,0,0,3", "-1.0,0,0,1,1,2", "1.0,1,1,1", "1.0,1,7,1", "1.0,1,13,1",
+                 "%f,1,6,1" % b, "%f,1,12,1" % b, "-1.0,1,0,2,1,1", "-1.0,1,1,3", "%f,1,0,3" % c,
+                 "%f,1,0,1,1,2" % c, "-%f,0,0,2,1,1" % c, "-%f,0,1,3" % c]
+    (tmp_path / "runcoupling.dat").write_text("\n".join(couplings) + "\n")
+    r = run(["-t", "0", "-d", "0.1", "-N", "16,16", "-L", "100,100", "-n", "2", "-v", "-D1",
+             "run"], tmp_path)
+    lines = r.stdout.split("\n")
+    assert "Using coupling terms from file" in lines
+    eq = [l for l in lines if l.startswith("u0'=") or l.startswith("u1'=")]
+    assert eq == GOLD["equations_2d"]
+
+
+def test_random_ic_is_glibc_rand(tmp_path):
+    """y = A0 + A/2*(2.0/RAND_MAX*rand()-1) after srand(seed) (rdcu.cu:898, 996)."""
+    run(["-t", "0", "-A", "1", "-s", "1", "-N", "4", "-n", "2", "run"], tmp_path)
+    y = np.fromfile(tmp_path / "runstates.dat")
+    assert y[:3].tolist() == [0.34018771715470952, -0.10561707318090696, 0.28309922375860586]
+    fs = np.fromfile(tmp_path / "runfs.dat")
+    assert fs.size == 8 + 2 and np.array_equal(fs[:8], y) and fs[8] == 0.0 and fs[9] == 0.01
+    # amplitude and offset: -A 0.5,3 -> values in [2.75, 3.25]
+    run(["-t", "0", "-A", "0.5,3", "-N", "64", "run2"], tmp_path)
+    y2 = np.fromfile(tmp_path / "run2states.dat")
+    assert y2.min() >= 2.75 and y2.max() <= 3.25
+
+
+def test_output_files_layout(tmp_path):
+    """-D3: times, steps per attempt, states/f per frame, Y complex per frame,
+    output times as products dt*nn (rdcu.cu:1052-1059)."""
+    n, N = 2, 64
+    run(["-n", 2, "-N", N, "-L", 30, "-t", 0.5, "-d", 0.1, "-A", 1, "-D3"] + C1 + ["o"], tmp_path)
+    t = np.fromfile(tmp_path / "otimes.dat")
+    assert t.tolist() == [0.0] + [0.1 * k for k in range(1, 6)]
+    nf = len(t)
+    assert np.fromfile(tmp_path / "ostates.dat").size == nf * n * N
+    assert np.fromfile(tmp_path / "of.dat").size == nf * n * N
+    Y = np.fromfile(tmp_path / "oY.dat").view(np.complex128).reshape(nf, 3, n, N)
+    s = np.fromfile(tmp_path / "ostates.dat").reshape(nf, n, N)
+    assert np.array_equal(Y[:, 0].real, s) and np.all(Y.imag == 0)
+    steps = np.fromfile(tmp_path / "osteps.dat")
+    out = (tmp_path / "o.out").read_text()
+    acc = int(out.split("steps: ")[1].split()[0])
+    assert steps.size >= acc > 0
+    # f frame = RHS of the state frame; the identity block is the state
+    fs = np.fromfile(tmp_path / "ofs.dat")
+    assert fs.size == n * N + 2 and fs[-2] >= 0.5
+
+
+def test_restart_is_bitwise_continuation(tmp_path):
+    """-R continues from FILEBASEfs.dat (rdcu.cu:944-990) and appends: the
+    files equal those of one uninterrupted run."""
+    base = ["-n", 2, "-N", 64, "-L", 30, "-d", 0.25, "-A", 1, "-D2"] + C1
+    a = tmp_path / "a"
+    b = tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    run(base + ["-t", 2.0, "x"], a)
+    run(base + ["-t", 1.0, "x"], b)
+    r = run(base + ["-t", 2.0, "-R", "x"], b)
+    assert "Reloading final conditions from file" in r.stdout
+    for suf in ("times.dat", "states.dat", "f.dat", "steps.dat", "fs.dat"):
+        assert (a / ("x" + suf)).read_bytes() == (b / ("x" + suf)).read_bytes(), suf
+
+
+def test_fixed_step_and_step_bounds(tmp_path):
+    run(["-n", 2, "-N", 32, "-L", 30, "-t", 0.05, "-d", "0.01,0.005", "-F", "-A", 1] + C1 + ["f"],
+        tmp_path)
+    steps = np.fromfile(tmp_path / "fsteps.dat")
+    t, cnt = 0.0, 0
+    while t < 0.05:  # no clip to t1: t = 10*0.005 rounds below 0.05, so 11 steps
+        t += 0.005
+        cnt += 1
+    assert steps.size == cnt == 11 and np.all(steps == 0.005)
+    # hmax caps accepted steps; hmin ends the loop
+    run(["-n", 2, "-N", 32, "-L", 30, "-t", 0.5, "-d", "0.1,0.001,0,0.002", "-A", 1] + C1 + ["m"],
+        tmp_path)
+    assert np.fromfile(tmp_path / "msteps.dat").max() <= 0.002
+    run(["-n", 2, "-N", 32, "-L", 30, "-t", 5, "-d", "1,0.5,0.4", "-A", 1] + C1 + ["h"], tmp_path)
+    fs = np.fromfile(tmp_path / "hfs.dat")
+    assert fs[-2] == 0.0  # first attempt rejected, h < hmin: no step taken
+
+
+def test_ic_file_and_append_mode(tmp_path):
+    n, N = 2, 32
+    y0 = np.linspace(-0.3, 0.3, n * N)
+    y0.tofile(tmp_path / "icic.dat")
+    r = run(["-n", n, "-N", N, "-L", 30, "-t", 0.2, "-d", 0.1] + C1 + ["ic"], tmp_path)
+    assert "Using initial conditions from file" in r.stdout
+    s = np.fromfile(tmp_path / "icstates.dat").reshape(-1, n * N)
+    assert np.array_equal(s[0], y0)
+    np.zeros(5).tofile(tmp_path / "badic.dat")
+    r = run(["-n", n, "-N", N, "-t", 0] + ["bad"], tmp_path)
+    assert "initial conditions file not compatible with N!" in r.stdout
+    assert "Using random initial conditions" in r.stdout
+
+
+def test_stiff_and_jacobian_output_rejected(tmp_path):
+    r = run(["-S", "x"], tmp_path, rc=2)
+    assert "-S" in r.stderr
+    r = run(["-D4", "x"], tmp_path, rc=2)
+    assert "-D4" in r.stderr
+
+
+def test_bad_factor_index_rejected(tmp_path):
+    r = run(["-n", 2, "-N", 16, "-c", "1.0,0,99,1", "x"], tmp_path, rc=2)
+    assert "out of range" in r.stderr

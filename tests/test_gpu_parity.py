@@ -41,6 +41,10 @@ RHS_CASES = [
     ("2d-512", [512, 512], [200.0, 200.0], "cgle"),
     ("3d-16x24x32", [16, 24, 32], [10.0, 20.0, 30.0], "mixed"),
     ("3d-32", [32, 32, 32], [30.0, 30.0, 30.0], "cgle"),
+    ("2d-64x192", [64, 192], [40.0, 120.0], "cgle"),      # x length 3*2^k: 24 values/thread
+    ("2d-96x384", [96, 384], [60.0, 240.0], "fhn"),
+    ("2d-40x20-lds", [40, 20], [30.0, 15.0], "mixed"),    # radix-5 lengths: LDS kernels
+    ("3d-10x12x8-lds", [10, 12, 8], [5.0, 6.0, 4.0], "cgle"),
 ]
 
 
@@ -67,6 +71,28 @@ def test_rhs_and_factors_bitwise(name, Ns, Ls, kind, hlib, olib, rng):
         fo, Yo = o.rhs(y, factors=True)
     assert np.array_equal(Yg.real, Yo.real), np.abs(Yg.real - Yo.real).max()
     assert np.array_equal(fg, fo), np.abs(fg - fo).max()
+
+
+@pytest.mark.parametrize("Ns,Ls,kind", [([64, 96], [40.0, 70.0], "mixed"),
+                                         ([16, 24, 32], [10.0, 20.0, 30.0], "mixed"),
+                                         ([256], [100.0], "cgle")])
+def test_register_and_lds_paths_identical(hlib, rng, monkeypatch, Ns, Ls, kind):
+    """The register-resident kernels (with stage fusion) and the LDS kernels
+    give identical RHS values and step sequences."""
+    prob = make_problem(Ns, Ls, kind)
+    y0 = rng.uniform(1.0, 2.0, prob.n * prob.N) if kind == "mixed" else \
+        glibc_rand_ic(prob.n, prob.N, seed=5, A=1.0)
+    res = []
+    for path in ("reg", "lds"):
+        monkeypatch.setenv("RDCU_PATH", path)
+        with Integrator(hlib, prob, y0) as g:
+            f = g.rhs(y0)
+            # bounded by attempts: the "mixed" test dictionary blows up in finite time
+            t, h, done = g.advance(0.0, 1e-3, max_attempts=30)
+            res.append((f, (t, h, done), g.state()))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1] == res[1][1] and res[0][1][2] == 30
+    assert np.array_equal(res[0][2], res[1][2], equal_nan=True)
 
 
 def test_rhs_plain_equals_factor_path(hlib, rng):
