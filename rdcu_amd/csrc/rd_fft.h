@@ -113,11 +113,15 @@ __device__ __forceinline__ void dft(cx *v) {
 constexpr int kMaxPass = 12;
 
 // Radix plan of one transform length, passed by value in kernel arguments.
+// The radix of pass i is (code >> 4i) & 15: a packed word rather than an
+// array, so that indexing it per pass stays in scalar registers instead of
+// a private-memory copy of the argument.
 struct FftPlan {
   int L;
   int np;
-  int rad[kMaxPass];
+  unsigned long long code;
   const cx *tw;  // T_L, L entries, device memory
+  __host__ __device__ int rad(int i) const { return (int)((code >> (4 * i)) & 15u); }
 };
 
 // One Stockham pass over `nl` lines in LDS.  Element e of line l lives at
@@ -168,7 +172,7 @@ __device__ cx *lds_fft(cx *a, cx *b, const FftPlan &p, int nl, int ls, int es, i
   cx *src = a, *dst = b;
   int Ns = 1;
   for (int ip = 0; ip < p.np; ++ip) {
-    const int R = p.rad[ip];
+    const int R = p.rad(ip);
     switch (R) {
       case 8: stockham_pass<8, DIR, LFAST>(src, dst, p.L, Ns, p.tw, nl, ls, es, tid, nt); break;
       case 4: stockham_pass<4, DIR, LFAST>(src, dst, p.L, Ns, p.tw, nl, ls, es, tid, nt); break;
@@ -238,7 +242,7 @@ template <int E, int DIR>
 __device__ void reg_fft(cx (&v)[E], cx *buf, int es, const cx *tw, const FftPlan &p, int t, int T) {
   int Ns = 1;
   for (int ip = 0; ip < p.np; ++ip) {
-    const int R = p.rad[ip];
+    const int R = p.rad(ip);
     if (ip > 0) {
       __syncthreads();
 #pragma unroll

@@ -106,6 +106,29 @@ __device__ __forceinline__ void stage_v(V *dst, const V *__restrict__ src, int n
   }
 }
 
+// Loads up to U*nt values into registers now and stores them to LDS later,
+// so the loads share one memory round trip with the data loads issued in
+// between (values past U*nt use a plain loop at store time).
+template <int U, class V>
+struct Prefetch {
+  V r[U];
+  __device__ __forceinline__ void load(const V *__restrict__ src, int n, int tid, int nt) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = tid + u * nt;
+      r[u] = i < n ? src[i] : V{};
+    }
+  }
+  __device__ __forceinline__ void store(V *dst, const V *__restrict__ src, int n, int tid, int nt) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = tid + u * nt;
+      if (i < n) dst[i] = r[u];
+    }
+    for (int i = tid + U * nt; i < n; i += nt) dst[i] = src[i];
+  }
+};
+
 // Multiplier value of derivative block blk at wavenumber indices nk with the
 // wavenumbers of each axis already loaded (kv) -- the same operations as
 // apply_mult.
@@ -146,7 +169,8 @@ __global__ __launch_bounds__(256) void k_rowf_reg(Grid g, FftPlan px, const cx *
   const int rr = tid / T, t = tid - rr * T;
   const long row = (long)blockIdx.x * rpb + rr;
   const bool act = row < nrows;
-  stage_cx<8>(tw, px.tw, L, tid, blockDim.x);
+  Prefetch<4, cx> pf_tw;
+  pf_tw.load(px.tw, L, tid, blockDim.x);
 
   cx v[E];
   if (act) {
@@ -188,6 +212,7 @@ __global__ __launch_bounds__(256) void k_rowf_reg(Grid g, FftPlan px, const cx *
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = cx{0.0, 0.0};
   }
+  pf_tw.store(tw, px.tw, L, tid, blockDim.x);
   cx *lb = buf + rr * padded_len(L);
   reg_fft<E, -1>(v, lb, 1, tw, px, t, T);
 #pragma unroll
@@ -242,15 +267,19 @@ __global__ __launch_bounds__(256) void k_col_reg(Grid g, FftPlan p, ColGeom cg, 
   const int c = tid % tc, t = tid / tc;
   const bool act = c < nc;
   cx *tw = smem, *buf = smem + L;
-  stage_cx<8>(tw, p.tw, L, tid, blockDim.x);
   // mode 1: wavenumbers of axis 0 after the forward spectrum in LDS
   double *k0s = reinterpret_cast<double *>(buf + tc * padded_len(L) + tc * L);
-  if (mode == 1) stage_v<8>(k0s, g.kt[0], L, tid, blockDim.x);
   const long lbase = (long)g2 * cg.gs2 + c0 + c;
   cx *src = (mode == 2 ? b.F : b.S) + (long)g1 * cg.gs1 + lbase;
   cx v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = act ? src[(long)(t + T * q) * cg.es] : cx{0.0, 0.0};
+  Prefetch<4, cx> pf_tw;
+  pf_tw.load(p.tw, L, tid, blockDim.x);
+  Prefetch<4, double> pf_k;
+  if (mode == 1) pf_k.load(g.kt[0], L, tid, blockDim.x);
+  pf_tw.store(tw, p.tw, L, tid, blockDim.x);
+  if (mode == 1) pf_k.store(k0s, g.kt[0], L, tid, blockDim.x);
   cx *lb = buf + c;
   if (mode != 1) {
     if (mode == 0) reg_fft<E, -1>(v, lb, tc, tw, p, t, T);
@@ -303,6 +332,36 @@ __global__ __launch_bounds__(256) void k_col_reg(Grid g, FftPlan p, ColGeom cg, 
 // with nx.ns > 0 the fused next stage (stage sum + forward x pass -> S).
 // Block: G*T threads, G >= n.  LDS: twiddles L + G*padded_len(L) complex,
 // then (n+nf)*Nx + n*Nx doubles, then the dictionary program.
+// Pre-processing of the half-complex -> real transform of factor line m at
+// row position r, elements t + T*q: the split step of the real inverse
+// transform (and in 1-D the multiplier, the factor spectra being S itself).
+template <int E>
+__device__ __forceinline__ void c2r_pre(const Grid &g, const FacList &fl, const Bufs &b,
+                                        const cx *__restrict__ twN, int m, long r, int t, int T,
+                                        cx (&v)[E]) {
+  const int L = g.L;
+  const int blk = fl.fblk[m];
+  const cx *row = g.ndim == 1 ? b.S + (long)fl.ffield[m] * g.Nh : b.F + (long)m * g.Nh + r * g.Nxh;
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int k = t + T * q;
+    cx Av = row[k], Bv = row[L - k];
+    if (g.ndim == 1) {
+      int nk0 = k, nk1 = L - k;
+      Av = apply_mult(g, blk, &nk0, Av);
+      Bv = apply_mult(g, blk, &nk1, Bv);
+    }
+    Bv = cconj(Bv);
+    if (k == 0) {
+      Av.im = 0.0;
+      Bv.im = 0.0;
+    }
+    const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
+    const cx tt = cmul(D, cconj(twN[k]));
+    v[q] = cx{Ev.re - tt.im, Ev.im + tt.re};
+  }
+}
+
 template <int E, class DE>
 __device__ __forceinline__ void rowi_body(Grid g, FftPlan px, const cx *__restrict__ twN,
                                           DevState *st, int check_active, Bufs b, FacList fl,
@@ -323,53 +382,45 @@ __device__ __forceinline__ void rowi_body(Grid g, FftPlan px, const cx *__restri
   double *fv = vals + (long)(n + nf) * Nx;
   int *prog = reinterpret_cast<int *>(fv + (long)n * Nx);
   cx *lb = buf + gl * PL;
-  stage_cx<8>(tw, px.tw, L, tid, nt);
-  if constexpr (DE::kProg) stage_v<8>(prog, dict.prog, dict.nwords, tid, nt);
-
   const double *src =
       src_kind == SRC_Y ? b.y[st->iy] : (src_kind == SRC_YT ? b.y[st->iyt] : src_ptr);
-  for (int x0 = tid; x0 < Nx; x0 += 4 * nt) {  // identity rows: all loads in flight first
-    double tmp[kNMax][4];
+  // Prologue: every global load is issued before the first LDS store or
+  // use, so they share one memory round trip (identity rows, twiddles,
+  // program, and for E = 8 the first factor group's pre-processed spectrum).
+  constexpr int IDU = 8;
+  double idr[IDU];
 #pragma unroll
-    for (int f = 0; f < kNMax; ++f)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int x = x0 + i * nt;
-        tmp[f][i] = (f < n && x < Nx) ? src[f * N + r * Nx + x] : 0.0;
-      }
-#pragma unroll
-    for (int f = 0; f < kNMax; ++f)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int x = x0 + i * nt;
-        if (f < n && x < Nx) vals[f * Nx + x] = tmp[f][i];
-      }
+  for (int u = 0; u < IDU; ++u) {
+    const int w = tid + u * nt;
+    idr[u] = w < n * Nx ? src[(w / Nx) * N + r * Nx + (w % Nx)] : 0.0;
   }
+  Prefetch<4, cx> pf_tw;
+  pf_tw.load(px.tw, L, tid, nt);
+  Prefetch<4, int> pf_p;
+  if constexpr (DE::kProg) pf_p.load(dict.prog, dict.nwords, tid, nt);
+  constexpr bool kPre = E <= 8;  // register budget: only for 8 values per thread
+  cx v0[kPre ? E : 1];
+  if constexpr (kPre) {
+    if (gl < nf) c2r_pre<E>(g, fl, b, twN, gl, r, t, T, v0);
+  }
+  pf_tw.store(tw, px.tw, L, tid, nt);
+  if constexpr (DE::kProg) pf_p.store(prog, dict.prog, dict.nwords, tid, nt);
+#pragma unroll
+  for (int u = 0; u < IDU; ++u) {
+    const int w = tid + u * nt;
+    if (w < n * Nx) vals[w] = idr[u];
+  }
+  for (int w = tid + IDU * nt; w < n * Nx; w += nt) vals[w] = src[(w / Nx) * N + r * Nx + (w % Nx)];
+
   for (int g0 = 0; g0 < nf; g0 += G) {
     const int m = g0 + gl;
     const bool act = m < nf;
     cx v[E];
-    if (act) {
-      const int blk = fl.fblk[m];
-      const cx *row = g.ndim == 1 ? b.S + (long)fl.ffield[m] * g.Nh : b.F + (long)m * g.Nh + r * Nxh;
+    if (act && kPre && g0 == 0) {
 #pragma unroll
-      for (int q = 0; q < E; ++q) {
-        const int k = t + T * q;
-        cx Av = row[k], Bv = row[L - k];
-        if (g.ndim == 1) {
-          int nk0 = k, nk1 = L - k;
-          Av = apply_mult(g, blk, &nk0, Av);
-          Bv = apply_mult(g, blk, &nk1, Bv);
-        }
-        Bv = cconj(Bv);
-        if (k == 0) {
-          Av.im = 0.0;
-          Bv.im = 0.0;
-        }
-        const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
-        const cx tt = cmul(D, cconj(twN[k]));
-        v[q] = cx{Ev.re - tt.im, Ev.im + tt.re};
-      }
+      for (int q = 0; q < E; ++q) v[q] = v0[kPre ? q : 0];
+    } else if (act) {
+      c2r_pre<E>(g, fl, b, twN, m, r, t, T, v);
     } else {
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = cx{0.0, 0.0};
@@ -475,8 +526,9 @@ __device__ __forceinline__ void rowi_body(Grid g, FftPlan px, const cx *__restri
       const double2 yv2 = *reinterpret_cast<const double2 *>(y + off);
       const double2 a1 = *reinterpret_cast<const double2 *>(ks[0] + off);
       double ar = nx.a[0] * a1.x, ai = nx.a[0] * a1.y;
-      for (int s = 1; s < nx.ns; ++s) {
-        if (nx.a[s] == 0.0) continue;
+#pragma unroll
+      for (int s = 1; s < 6; ++s) {  // unrolled: ks[] stays in registers
+        if (s >= nx.ns || nx.a[s] == 0.0) continue;
         double kx_, ky_;
         if (s == nx.ns - 1) {  // the stage just computed
           kx_ = fv[f * Nx + j2];

@@ -189,7 +189,8 @@ struct rdcu_ctx {
                                          " must have only the prime factors 2, 3, 5");
     p.L = len;
     p.np = (int)rad.size();
-    for (int i = 0; i < p.np; ++i) p.rad[i] = rad[i];
+    p.code = 0;
+    for (int i = 0; i < p.np; ++i) p.code |= (unsigned long long)rad[i] << (4 * i);
     std::vector<cx> tw(len);
     for (int m = 0; m < len; ++m) tw[m] = unit_root(m, len);
     cx *d = nullptr;
