@@ -43,6 +43,28 @@ def algorithmic_bytes(n, M, Ns):
     return {"row_r2c": fwd, "row_fwd": fwd, "col_mult": col, "row_c2r": inv, "row_inv": inv}
 
 
+def copy_calibration(mib, reps):
+    """A plain device-to-device copy kernel (torch's elementwise copy) of a known
+    size: the FETCH_SIZE / WRITE_SIZE counters it reads, against its exact
+    byte count, give the scale factor applied to the RHS kernels' counters."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.arange(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"mode": "copy", "bytes_read": n * 8, "bytes_written": n * 8,
+                      "ms": ms, "GBps": 2 * n * 8 / (ms * 1e-3) / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,12 +74,20 @@ def main():
     ap.add_argument("--reps", type=int, default=200, help="RHS evaluations per kernel timing")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall-clock budget of the CPU-oracle sample (0 to skip)")
+    ap.add_argument("--mode", choices=("bench", "rhs", "copy"), default="bench",
+                    help="bench: the JSON line; rhs: only the per-kernel RHS timing (a short "
+                         "target for profilers); copy: a plain device copy of --copy-mib MiB, "
+                         "the calibration for the HBM byte counters")
+    ap.add_argument("--copy-mib", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each instance lives "
               "on one GPU; see DESIGN.md for how a run would shard). Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.mode == "copy":
+        return copy_calibration(args.copy_mib, args.reps)
 
     import numpy as np
     from rdcu_amd import Integrator, Problem, cgle_terms, glibc_rand_ic, hip_lib, load
@@ -68,6 +98,13 @@ def main():
     y0 = glibc_rand_ic(2, G * G, seed=1, A=1.0)
     lib = hip_lib()
     it = Integrator(lib, prob, y0, atol=1e-8, rtol=1e-8)
+
+    if args.mode == "rhs":
+        km = it.time_rhs(args.reps)
+        it.close()
+        print(json.dumps({"mode": "rhs", "grid": Ns, "reps": args.reps, "kernels_ms": km}),
+              flush=True)
+        return
 
     t, h = 0.0, 0.01  # -d 1: first step dt/100
     t, h, done = it.advance(t, h, max_attempts=args.warmup)
