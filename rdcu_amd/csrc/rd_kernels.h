@@ -277,7 +277,15 @@ struct ColGeom {
   int tc;      // columns per tile
   int axis;    // grid axis of the lines (multiplier index)
   int nparts;  // blocks per tile in mode 1 (factor split, register path)
+  int nblk;    // logical blocks (register path; the grid is rounded up to 8)
 };
+
+// Logical block of a launch whose grid is a multiple of the 8 XCDs: hardware
+// hands block b to XCD b % 8, so logical blocks [x*nb/8, (x+1)*nb/8) -- which
+// read neighbouring columns and the same spectra -- share XCD x's L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  return (b & 7) * (nb >> 3) + (b >> 3);
+}
 
 __global__ void k_col(Grid g, FftPlan p, ColGeom cg, DevState *st, int check_active, Bufs b,
                       FacList fl, int mode) {

@@ -249,8 +249,10 @@ __global__ __launch_bounds__(256) void k_col_reg(Grid g, FftPlan p, ColGeom cg, 
   // mode 1: blocks come in groups of nparts, each transforming up to fpw of
   // the field's factors (the forward transform is repeated per part)
   const int nparts = mode == 1 ? cg.nparts : 1;
-  const int part = blockIdx.x % nparts;
-  const int tile = blockIdx.x / nparts;
+  const int lblk = xcd_block(blockIdx.x, gridDim.x);
+  if (lblk >= cg.nblk) return;
+  const int part = lblk % nparts;
+  const int tile = lblk / nparts;
   const int grp = tile / cg.ntc, ct = tile - grp * cg.ntc;
   const int g1 = grp / cg.gdiv, g2 = grp - g1 * cg.gdiv;
   int m0 = 0, m1 = 0;

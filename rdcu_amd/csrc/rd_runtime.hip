@@ -329,7 +329,8 @@ void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups
   const int fpw = std::max(1, c->cr_fpw);
   cg.nparts = mode == 1 ? (c->max_fpf + fpw - 1) / fpw : 1;
   if (cg.nparts < 1) cg.nparts = 1;
-  dim3 g((unsigned)(ngroups * cg.ntc * cg.nparts));
+  cg.nblk = (int)(ngroups * cg.ntc * cg.nparts);
+  dim3 g((unsigned)((cg.nblk + 7) / 8 * 8));
   size_t lds = (size_t)(cg.L + cg.tc * rd::padded_len(cg.L) + (mode == 1 ? cg.tc * cg.L : 0)) *
                    sizeof(cx) +
                (mode == 1 ? (size_t)cg.L * sizeof(double) : 0);
@@ -589,6 +590,37 @@ void build_jit(rdcu_ctx *c) {
 // (the arithmetic does not depend on the tiling), so the fastest is kept.
 // Candidates are timed alone with HIP events on the plain RHS of the
 // current state.
+// Shapes given by the caller instead of the autotuner (profiling runs that
+// must see only the launches of one shape), checked against the same limits.
+int fixed_shapes(rdcu_ctx *c, const char *spec) {
+  int rpb = 0, tc = 0, fpw = 0, G = 0;
+  if (sscanf(spec, "%d,%d,%d,%d", &rpb, &tc, &fpw, &G) != 4)
+    return fail(RDCU_EINVAL, "RDCU_SHAPES must be rpb,tc,fpw,G");
+  const int T = c->L / c->Ex;
+  int Lmax = c->Ns[0], Emin = c->E0;
+  if (c->ndim == 3) {
+    Lmax = std::max(Lmax, c->Ns[1]);
+    Emin = std::min(c->E0, c->E1);
+  }
+  const size_t ri_lds = (size_t)(c->L + G * rd::padded_len(c->L)) * sizeof(cx) +
+                        (size_t)(2 * c->n + c->nf_used) * c->Nx * 8 + (size_t)c->dictp.nwords * 4;
+  const bool ok =
+      rpb >= 1 && rpb * T <= 256 &&
+      (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx) <= 160 * 1024 &&
+      (c->ndim < 2 || (tc >= 1 && tc * (Lmax / Emin) <= 256 &&
+                       (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) <=
+                           160 * 1024 &&
+                       fpw >= 1 && fpw <= std::max(1, c->max_fpf))) &&
+      G >= c->n && G <= std::max(c->n, c->nf_used) && G * T <= 256 && ri_lds <= 160 * 1024;
+  if (!ok) return fail(RDCU_EINVAL, std::string("RDCU_SHAPES out of range: ") + spec);
+  c->rf_rpb = rpb;
+  c->cr_tc = tc;
+  c->cr_fpw = fpw;
+  c->ri_G = G;
+  c->tune_log = std::string("fixed ") + spec;
+  return RDCU_OK;
+}
+
 int autotune(rdcu_ctx *c) {
   hipEvent_t e0, e1;
   HIPCHECK(hipEventCreate(&e0));
@@ -1064,10 +1096,16 @@ int rdcu_create(const rdcu_problem *p, double atol, double rtol, int fixedstep, 
       lds_raised = true;
     }
     build_jit(c.get());
-    if ((st = autotune(c.get()))) return st;
+    const char *shp = getenv("RDCU_SHAPES");  // "rpb,tc,fpw,G": fixed shapes, no tuning
+    if (shp && *shp) {
+      if ((st = fixed_shapes(c.get(), shp))) return st;
+    } else if ((st = autotune(c.get()))) {
+      return st;
+    }
     const char *vb = getenv("RDCU_VERBOSE");
     if (vb && *vb && *vb != '0')
-      fprintf(stderr, "rdcu jit: %s\nrdcu autotune: %s\n", c->jit_log.c_str(), c->tune_log.c_str());
+      fprintf(stderr, "rdcu jit: %s\nrdcu autotune: %s\nrdcu shapes: %d,%d,%d,%d\n",
+              c->jit_log.c_str(), c->tune_log.c_str(), c->rf_rpb, c->cr_tc, c->cr_fpw, c->ri_G);
   }
 
   // capture one attempt
