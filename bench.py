@@ -43,6 +43,28 @@ def algorithmic_bytes(n, M, Ns):
     return {"row_r2c": fwd, "row_fwd": fwd, "col_mult": col, "row_c2r": inv, "row_inv": inv}
 
 
+PROFILE = os.path.join(ROOT, "profiles", "rhs_2d_cgle_512.md")
+KERNEL_SYMBOL = {"row_fwd": "k_rowf_reg", "col_mult": "k_col_reg", "row_inv": "rdcu_rowi_jit",
+                 "row_r2c": "k_row_r2c", "col": "k_col", "row_c2r": "k_row_c2r"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration),
+    or None when the summary has no row for it."""
+    sym = KERNEL_SYMBOL.get(kernel)
+    if not sym or not os.path.exists(PROFILE):
+        return None
+    for line in open(PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and sym in cells[0]:
+            try:
+                return float(cells[5]) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
 def copy_calibration(mib, reps):
     """A plain device-to-device copy kernel (torch's elementwise copy) of a known
     size: the FETCH_SIZE / WRITE_SIZE counters it reads, against its exact
@@ -146,7 +168,7 @@ def main():
                    "grid": Ns, "attempts_per_s": attempts / el, "accepted": accepted,
                    "t_end": t, "h_end": h, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": None,
+                     "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": measured_traffic(dom) if G == 512 else None,
                      "bytes_per_launch": kb[dom], "ms_per_launch": km[dom]},
         "rhs": {"ms": km["rhs"], "bytes": B_rhs, "achieved_GBps": rhs_ach / 1e9,
                 "frac": rhs_ach / PEAK_HBM, "kernels_ms": km, "kernel_bytes": kb,
