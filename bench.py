@@ -33,14 +33,19 @@ def algorithmic_bytes(n, M, Ns):
     half-spectrum storage (8 B per grid point per field or factor) with one
     read and one write per array per pass:
       x forward   read u (8nN), write n half spectra (8nN)
-      axis 0      read n spectra (8nN), write M factor spectra (8MN)
+      y forward   (3-D) read and write the n spectra (16nN)
+      last axis   read n spectra (8nN), write M factor spectra (8MN)
+      y inverse   (3-D) read and write the M factor spectra (16MN)
       x inverse   read M factor spectra (8MN) and u (8nN), write f (8nN)
-    (2-D; the sum is B_rhs = 8N[n(2*ndim+1) + 2M(ndim-1)])."""
+    The sum is B_rhs = 8N[n(2*ndim+1) + 2M(ndim-1)]."""
     N = 1
     for s in Ns:
         N *= s
     fwd, col, inv = 16 * n * N, 8 * n * N + 8 * M * N, 8 * M * N + 16 * n * N
-    return {"row_r2c": fwd, "row_fwd": fwd, "col_mult": col, "row_c2r": inv, "row_inv": inv}
+    out = {"row_r2c": fwd, "row_fwd": fwd, "col_mult": col, "row_c2r": inv, "row_inv": inv}
+    if len(Ns) == 3:
+        out.update({"col_fwd": 16 * n * N, "col_inv": 16 * M * N})
+    return out
 
 
 PROFILE = os.path.join(ROOT, "profiles", "rhs_2d_cgle_512.md")
@@ -90,10 +95,16 @@ def copy_calibration(mib, reps):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600, help="timed DP45 attempts")
-    ap.add_argument("--warmup", type=int, default=60, help="untimed attempts first")
-    ap.add_argument("--grid", type=int, default=512, help="grid points per axis")
-    ap.add_argument("--reps", type=int, default=200, help="RHS evaluations per kernel timing")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed DP45 attempts (default 600 in 2-D, 20 in 3-D)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed attempts first (default 60 in 2-D, 3 in 3-D)")
+    ap.add_argument("--grid", type=int, default=None, help="grid points per axis")
+    ap.add_argument("--workload", choices=("cgle2d", "cgle3d"), default="cgle2d",
+                    help="cgle2d: 2-D CGLE 512x512, L=200, amplitude 1 (the headline); "
+                         "cgle3d: 3-D CGLE 256^3, L=100, amplitude 0.1 (same coefficients)")
+    ap.add_argument("--reps", type=int, default=None,
+                    help="RHS evaluations per kernel timing (default 200 in 2-D, 10 in 3-D)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall-clock budget of the CPU-oracle sample (0 to skip)")
     ap.add_argument("--mode", choices=("bench", "rhs", "copy"), default="bench",
@@ -114,10 +125,23 @@ def main():
     import numpy as np
     from rdcu_amd import Integrator, Problem, cgle_terms, glibc_rand_ic, hip_lib, load
 
-    G = args.grid
-    Ns, Ls = [G, G], [200.0, 200.0]
-    prob = Problem(2, Ns, Ls, cgle_terms(2, 2.0, 0.8))
-    y0 = glibc_rand_ic(2, G * G, seed=1, A=1.0)
+    nd = 2 if args.workload == "cgle2d" else 3
+    G = args.grid or (512 if nd == 2 else 256)
+    Ns, Ls = [G] * nd, [200.0 if nd == 2 else 100.0] * nd
+    amp = 1.0 if nd == 2 else 0.1
+    if args.steps is None:
+        args.steps = 600 if nd == 2 else 20
+    if args.warmup is None:
+        args.warmup = 60 if nd == 2 else 3
+    if args.reps is None:
+        args.reps = 200 if nd == 2 else 10
+
+    prob = Problem(2, Ns, Ls, cgle_terms(nd, 2.0, 0.8))
+    gname = "x".join(str(x) for x in Ns)
+    workload = (f"{nd}-D CGLE {gname}, L={Ls[0]:g}^{nd}, b=2, c=0.8, 2 FP64 fields, "
+                f"{len(prob.terms)}-term dictionary, IC amplitude {amp:g} seed 1, "
+                "rtol=atol=1e-8, DP45 pseudospectral")
+    y0 = glibc_rand_ic(2, G ** nd, seed=1, A=amp)
     lib = hip_lib()
     it = Integrator(lib, prob, y0, atol=1e-8, rtol=1e-8)
 
@@ -143,7 +167,7 @@ def main():
     # RHS kernels, HIP events on the library's stream
     km = it.time_rhs(args.reps)
     log("kernel ms:", json.dumps(km))
-    M = 4  # referenced derivative factors: d2/dy2 and d2/dx2 of both fields
+    M = 2 * nd  # referenced derivative factors: d2/dx_a^2 of both fields
     kb = {k: v for k, v in algorithmic_bytes(2, M, Ns).items() if k in km}
     B_rhs = sum(kb.values())
     dom = max((k for k in km if k != "rhs"), key=lambda k: km[k])
@@ -162,13 +186,13 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (glibc rand() initial condition, seed 1, amplitude 1)",
-        "config": {"workload": f"2-D CGLE {G}x{G}, L=200x200, b=2, c=0.8, 2 FP64 fields, "
-                               "18-term dictionary, rtol=atol=1e-8, DP45 pseudospectral",
+        "data": f"synthetic (glibc rand() initial condition, seed 1, amplitude {amp:g})",
+        "config": {"workload": workload,
                    "grid": Ns, "attempts_per_s": attempts / el, "accepted": accepted,
                    "t_end": t, "h_end": h, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": ach / PEAK_HBM, "traffic": measured_traffic(dom) if G == 512 else None,
+                     "unit": "GB/s", "frac": ach / PEAK_HBM,
+                     "traffic": measured_traffic(dom) if (nd, G) == (2, 512) else None,
                      "bytes_per_launch": kb[dom], "ms_per_launch": km[dom]},
         "rhs": {"ms": km["rhs"], "bytes": B_rhs, "achieved_GBps": rhs_ach / 1e9,
                 "frac": rhs_ach / PEAK_HBM, "kernels_ms": km, "kernel_bytes": kb,
@@ -190,7 +214,7 @@ def main():
             cel = time.perf_counter() - c0
         result["cpu_baseline"] = {
             "value": nacc / cel, "unit": "accepted DP45 steps/s", "cores": cores, "kind": "port",
-            "sample": f"first {na} DP45 attempts ({nacc} accepted) of the same 512x512 CGLE "
+            "sample": f"first {na} DP45 attempts ({nacc} accepted) of the same {gname} CGLE "
                       f"run from the same initial condition, FP64 C++ oracle, OpenMP "
                       f"{cores} threads, {cel:.1f} s",
             "attempts_per_s": na / cel}
