@@ -123,6 +123,37 @@ int rdcu_set_state(rdcu_ctx *ctx, const double *y);
  * uninitialised memory in block 0). */
 int rdcu_rhs(rdcu_ctx *ctx, const double *y, double *f_out, double *Y_out);
 
+/* ---- Batches: many independent problems integrated side by side ----------
+ * The reference runs a parameter sweep as one process per instance (a SLURM
+ * job array, sweep.sh:21-41, each running makelines.py:260-268 -> ./rdcu).
+ * A batch holds `count` problems (any mix of 1-D/2-D grids and dictionaries;
+ * transform lengths 2^k or 3*2^k) with initial states y0[i][n_i*N_i].  Every
+ * instance has its own t, h and step history, computed with exactly the
+ * arithmetic of a single rdcu_ctx on the same problem. */
+typedef struct rdcu_batch rdcu_batch;
+
+int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const *y0,
+                      double atol, double rtol, int fixedstep, int device, rdcu_batch **out);
+
+int rdcu_batch_destroy(rdcu_batch *batch);
+
+/* One attempt of every instance from (t[i], h[i]), as rdcu_step (dp45.cu:87-147)
+ * for each; accepted[i] = 1 or 0. */
+int rdcu_batch_step(rdcu_batch *batch, double *t, double *h, double hmin, double hmax,
+                    int *accepted);
+
+/* Runs every instance while (t[i] < t1[i] && h[i] > hmin), at most max_attempts
+ * attempts each, as rdcu_advance does for one (dp45.cu:154-169, k1 = f(y) first).
+ * attempts[i] (may be NULL) receives the attempts run by instance i. */
+int rdcu_batch_run(rdcu_batch *batch, double *t, double *h, double hmin, double hmax,
+                   const double *t1, long long max_attempts, long long *attempts);
+
+/* Instance i as an ordinary handle for rdcu_get_state, rdcu_set_state,
+ * rdcu_eval, rdcu_rhs, rdcu_get_stats.  It belongs to the batch (do not pass it
+ * to rdcu_destroy) and steps only through the batch (rdcu_step, rdcu_run and
+ * rdcu_advance return RDCU_ESTATE for it). */
+int rdcu_batch_member(rdcu_batch *batch, int i, rdcu_ctx **out);
+
 /* Counters: out[0] attempts, out[1] accepted, out[2] RHS evaluations,
  * out[3] host synchronisations (chunks). */
 int rdcu_get_stats(rdcu_ctx *ctx, long long out[4]);

@@ -666,6 +666,16 @@ struct rdcu_ctx {
   bool k1_valid = false, eval_valid = false;
   double last_norm = 0;
   long long attempts = 0, accepted = 0;
+  bool member = false;  // owned by a batch, stepped only through it
+};
+
+// A batch is its instances stepped one after another (the sweep of
+// sweep.sh:21-41 without the process per instance).
+struct rdcu_batch {
+  std::vector<rdcu_ctx *> m;
+  ~rdcu_batch() {
+    for (rdcu_ctx *c : m) delete c;
+  }
 };
 
 namespace {
@@ -793,12 +803,14 @@ int rdcu_destroy(rdcu_ctx *c) {
 int rdcu_step(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, int *accepted) {
   (void)hmin;
   if (!c || !t || !h || !accepted) return fail(RDCU_EINVAL, "null argument");
+  if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
   return attempt(c, t, h, hmax, accepted);
 }
 
 int rdcu_run(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double t1, double stop,
              rdcu_step_fn fn, void *user) {
   if (!c || !t || !h) return fail(RDCU_EINVAL, "null argument");
+  if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
   c->E.rhs(c->y.data(), c->k[0].data());  // dp45.cu:157
   c->k1_valid = true;
   while (*t < t1 && *h > hmin) {  // dp45.cu:159
@@ -810,9 +822,10 @@ int rdcu_run(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double
   return RDCU_OK;
 }
 
-int rdcu_advance(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double t1,
-                 long long max_attempts, long long *done) {
-  if (!c || !t || !h) return fail(RDCU_EINVAL, "null argument");
+// (a name of its own: this sits in the extern "C" block, where "advance"
+// would bind to libc's regexp function of that name)
+static int run_attempts(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double t1,
+                        long long max_attempts, long long *done) {
   long long k = 0;
   c->E.rhs(c->y.data(), c->k[0].data());
   c->k1_valid = true;
@@ -823,6 +836,66 @@ int rdcu_advance(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, do
     ++k;
   }
   if (done) *done = k;
+  return RDCU_OK;
+}
+
+int rdcu_advance(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double t1,
+                 long long max_attempts, long long *done) {
+  if (!c || !t || !h) return fail(RDCU_EINVAL, "null argument");
+  if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
+  return run_attempts(c, t, h, hmin, hmax, t1, max_attempts, done);
+}
+
+int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const *y0, double atol,
+                      double rtol, int fixedstep, int device, rdcu_batch **out) {
+  if (!out) return fail(RDCU_EINVAL, "null output handle");
+  *out = nullptr;
+  if (!probs || count < 1) return fail(RDCU_EINVAL, "a batch needs at least one problem");
+  std::unique_ptr<rdcu_batch> b(new rdcu_batch());
+  for (int i = 0; i < count; ++i) {
+    if (probs[i].ndim > 2) return fail(RDCU_EUNSUPPORTED, "batched instances are 1-D or 2-D");
+    rdcu_ctx *c = nullptr;
+    int st = rdcu_create(&probs[i], atol, rtol, fixedstep, y0 ? y0[i] : nullptr, device, &c);
+    if (st) return fail(st, "instance " + std::to_string(i) + ": " + g_err);
+    c->member = true;
+    b->m.push_back(c);
+  }
+  *out = b.release();
+  return RDCU_OK;
+}
+
+int rdcu_batch_destroy(rdcu_batch *b) {
+  delete b;
+  return RDCU_OK;
+}
+
+int rdcu_batch_member(rdcu_batch *b, int i, rdcu_ctx **out) {
+  if (!b || !out) return fail(RDCU_EINVAL, "null argument");
+  if (i < 0 || i >= (int)b->m.size()) return fail(RDCU_EINVAL, "instance index out of range");
+  *out = b->m[i];
+  return RDCU_OK;
+}
+
+int rdcu_batch_step(rdcu_batch *b, double *t, double *h, double hmin, double hmax, int *accepted) {
+  (void)hmin;
+  if (!b || !t || !h || !accepted) return fail(RDCU_EINVAL, "null argument");
+  for (size_t i = 0; i < b->m.size(); ++i) {
+    int st = attempt(b->m[i], &t[i], &h[i], hmax, &accepted[i]);
+    if (st) return st;
+  }
+  return RDCU_OK;
+}
+
+int rdcu_batch_run(rdcu_batch *b, double *t, double *h, double hmin, double hmax, const double *t1,
+                   long long max_attempts, long long *attempts) {
+  if (!b || !t || !h || !t1) return fail(RDCU_EINVAL, "null argument");
+  if (max_attempts < 0) return fail(RDCU_EINVAL, "max_attempts must be >= 0");
+  for (size_t i = 0; i < b->m.size(); ++i) {
+    long long k = 0;
+    int st = run_attempts(b->m[i], &t[i], &h[i], hmin, hmax, t1[i], max_attempts, &k);
+    if (st) return st;
+    if (attempts) attempts[i] = k;
+  }
   return RDCU_OK;
 }
 

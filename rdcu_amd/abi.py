@@ -31,6 +31,8 @@ SYMBOLS = (
     "rdcu_create", "rdcu_destroy", "rdcu_step", "rdcu_run", "rdcu_advance", "rdcu_time_rhs",
     "rdcu_eval", "rdcu_get_state", "rdcu_set_state", "rdcu_rhs", "rdcu_get_stats",
     "rdcu_last_error", "rdcu_backend",
+    "rdcu_batch_create", "rdcu_batch_destroy", "rdcu_batch_step", "rdcu_batch_run",
+    "rdcu_batch_member",
 )
 
 
@@ -81,6 +83,15 @@ def load(path: str) -> ctypes.CDLL:
     lib.rdcu_set_state.argtypes = [vp, _DP]
     lib.rdcu_rhs.argtypes = [vp, _DP, _DP, _DP]
     lib.rdcu_get_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+    lib.rdcu_batch_create.argtypes = [ctypes.POINTER(_Problem), ctypes.c_int,
+                                      ctypes.POINTER(_DP), ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    lib.rdcu_batch_destroy.argtypes = [vp]
+    lib.rdcu_batch_step.argtypes = [vp, _DP, _DP, ctypes.c_double, ctypes.c_double,
+                                    ctypes.POINTER(ctypes.c_int)]
+    lib.rdcu_batch_run.argtypes = [vp, _DP, _DP, ctypes.c_double, ctypes.c_double, _DP,
+                                   ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)]
+    lib.rdcu_batch_member.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
     lib.rdcu_last_error.restype = ctypes.c_char_p
     lib.rdcu_backend.restype = ctypes.c_char_p
     for s in SYMBOLS:
@@ -285,6 +296,98 @@ class Integrator:
         out = (ctypes.c_longlong * 4)()
         self._check(self.lib.rdcu_get_stats(self.h, out))
         return dict(attempts=out[0], accepted=out[1], rhs=out[2], syncs=out[3])
+
+
+class _Member(Integrator):
+    """Instance i of a Batch: state, eval, rhs and stats of an ordinary
+    handle; owned by the batch (closing it does nothing)."""
+
+    def __init__(self, batch: "Batch", i: int):
+        self.lib = batch.lib
+        self.prob = batch.probs[i]
+        self.nN = self.prob.n * self.prob.N
+        h = ctypes.c_void_p()
+        self._check(self.lib.rdcu_batch_member(batch.h, i, ctypes.byref(h)))
+        self.h = h
+        self._batch = batch
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """Many independent problems integrated side by side (rdcu_batch_*): the
+    in-process replacement of a sweep.sh job array (sweep.sh:21-41)."""
+
+    def __init__(self, lib: ctypes.CDLL, probs: Sequence[Problem], y0s: Sequence[np.ndarray],
+                 atol=1e-8, rtol=1e-8, fixed=False, device=0):
+        self.lib = lib
+        self.probs = list(probs)
+        if len(y0s) != len(self.probs):
+            raise ValueError("one initial state per problem")
+        cs, keeps, ys = [], [], []
+        for p, y in zip(self.probs, y0s):
+            y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+            if y.size != p.n * p.N:
+                raise ValueError(f"y0 has {y.size} values, expected {p.n * p.N}")
+            c, keep = p._c()
+            cs.append(c)
+            keeps.append(keep)
+            ys.append(y)
+        arr = (_Problem * len(cs))(*cs)
+        yp = (_DP * len(ys))(*[_ptr(y) for y in ys])
+        h = ctypes.c_void_p()
+        self._check(lib.rdcu_batch_create(arr, len(cs), yp, atol, rtol, int(bool(fixed)), device,
+                                          ctypes.byref(h)))
+        del keeps, ys
+        self.h = h
+        self.count = len(cs)
+
+    def _check(self, code):
+        if code != OK:
+            raise RdcuError(code, self.lib.rdcu_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rdcu_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __len__(self):
+        return self.count
+
+    def member(self, i: int) -> Integrator:
+        return _Member(self, i)
+
+    def step(self, t, h, hmin=0.0, hmax=np.inf):
+        """One attempt of every instance: returns (t, h, accepted) arrays."""
+        tt = np.array(t, dtype=np.float64, copy=True).reshape(-1)
+        hh = np.array(h, dtype=np.float64, copy=True).reshape(-1)
+        acc = (ctypes.c_int * self.count)()
+        self._check(self.lib.rdcu_batch_step(self.h, _ptr(tt), _ptr(hh), hmin, hmax, acc))
+        return tt, hh, np.array(acc[:], dtype=bool)
+
+    def run(self, t, h, t1, max_attempts=1 << 40, hmin=0.0, hmax=np.inf):
+        """Every instance while t < t1 and h > hmin: returns (t, h, attempts)."""
+        tt = np.array(t, dtype=np.float64, copy=True).reshape(-1)
+        hh = np.array(h, dtype=np.float64, copy=True).reshape(-1)
+        t1 = np.ascontiguousarray(np.broadcast_to(np.asarray(t1, dtype=np.float64),
+                                                  (self.count,)))
+        na = (ctypes.c_longlong * self.count)()
+        self._check(self.lib.rdcu_batch_run(self.h, _ptr(tt), _ptr(hh), hmin, hmax, _ptr(t1),
+                                            int(max_attempts), na))
+        return tt, hh, np.array(na[:], dtype=np.int64)
 
 
 def glibc_rand_ic(n: int, N: int, seed: int = 1, A: float = 1e-3, A0: float = 0.0) -> np.ndarray:

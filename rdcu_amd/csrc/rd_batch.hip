@@ -1,0 +1,608 @@
+// rd_batch.hip -- batched integration of many independent problems on one GPU
+// (included at the end of rd_runtime.hip: same translation unit).
+//
+// The reference runs a parameter sweep as one process per instance
+// (sweep.sh:21-41 -> makelines.py:260-268 -> ./rdcu).  Here the instances of
+// a sweep live side by side in HBM and every DP45 attempt of all of them is
+// one hipGraph: per shape class (same grid, same transform lengths) one
+// launch per pass walks the (instance, block) items of the whole class, with
+// a device table of per-instance buffers, state, factor lists and
+// dictionaries; step control then runs for every instance at once.  Each
+// instance keeps its own t, h and accept/reject history: an instance whose
+// attempts have ended (t >= t1, h <= hmin) is skipped by every launch, and the
+// arithmetic per instance is the single-instance sequence, so its steps are
+// bit-identical to a run of that problem alone.
+
+struct BatchGroup {  // instances of one class sharing a dictionary structure
+  int i0 = 0, i1 = 0;  // range in the class's table
+  hipFunction_t fn = nullptr;  // specialised x-inverse kernel (null: interpreter)
+};
+
+struct BatchClass {
+  std::vector<int> members;  // batch member indices, in table order
+  std::vector<BatchGroup> groups;
+  int off = 0;               // offset of this class in the batch's instance table
+  rdcu_ctx *c0 = nullptr;    // a representative member (geometry, plans, tables)
+  int Ex = 0, E0 = 0, rows = 0, nf_max = 0, nwords_max = 0, max_fpf = 1;
+  int rpb = 1, tc = 1, fpw = 1, G = 1;
+  long grid_rf = 0, grid_col = 0;  // persistent grid sizes (0: one block per item)
+  std::string tune_log;
+};
+
+struct rdcu_batch {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  std::vector<std::unique_ptr<rdcu_ctx>> members;
+  std::vector<int> pos;  // member -> position in the instance table
+  std::vector<BatchClass> classes;
+  rd::Inst *d_inst = nullptr;
+  rd::DevState *d_st = nullptr, *h_st = nullptr;  // instance table order
+  int *d_done = nullptr, *h_done = nullptr;
+  double *d_t = nullptr, *d_h = nullptr, *d_t1 = nullptr, *h_buf = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipModule_t jit_mod = nullptr;
+  std::string jit_log = "off";
+  bool k1_valid = false;
+  int fixed = 0;
+  long long nsync = 0;
+
+  ~rdcu_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (jit_mod) (void)hipModuleUnload(jit_mod);
+    members.clear();  // before the stream and state they borrow
+    for (void *p : {(void *)d_inst, (void *)d_st, (void *)d_done, (void *)d_t, (void *)d_h,
+                    (void *)d_t1})
+      if (p) (void)hipFree(p);
+    if (h_st) (void)hipHostFree(h_st);
+    if (h_done) (void)hipHostFree(h_done);
+    if (h_buf) (void)hipHostFree(h_buf);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int ninst(const rdcu_batch *b) { return (int)b->members.size(); }
+
+size_t rowi_lds(const BatchClass &cl, int G) {
+  const rdcu_ctx *c = cl.c0;
+  return (size_t)(c->L + G * rd::padded_len(c->L)) * sizeof(cx) +
+         (size_t)(2 * c->n + cl.nf_max) * c->Nx * sizeof(double) + (size_t)cl.nwords_max * 4;
+}
+
+template <int E>
+void launch_rowf_b(rdcu_batch *b, const BatchClass &cl, int src_kind, rd::StageCoef sc, int rpb,
+                   long grid, int check) {
+  const rdcu_ctx *c = cl.c0;
+  const int T = c->L / E;
+  const int per = (int)(((long)c->n * c->rows + rpb - 1) / rpb);
+  const long total = (long)cl.members.size() * per;
+  const long g = grid > 0 ? std::min(grid, total) : total;
+  const size_t lds = (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx);
+  hipLaunchKernelGGL(rd::k_rowf_b<E>, dim3((unsigned)g), dim3(rpb * T), lds, b->stream,
+                     b->d_inst + cl.off, (int)cl.members.size(), per, c->px, c->twN, src_kind,
+                     sc, rpb, check);
+}
+
+template <int E>
+void launch_col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, long grid, int check) {
+  const rdcu_ctx *c = cl.c0;
+  rd::ColGeom cg{};
+  cg.L = c->Ns[0];
+  cg.ntc = (c->Nxh + tc - 1) / tc;
+  cg.tc = tc;
+  cg.gs1 = c->Nh;
+  cg.es = c->Nxh;
+  cg.gdiv = 1;
+  cg.nparts = (cl.max_fpf + fpw - 1) / fpw;
+  const int per = c->n * cg.ntc * cg.nparts;
+  cg.nblk = per;
+  const int T = cg.L / E;
+  (void)grid;
+  const long total = (long)cl.members.size() * per;
+  const long g = (total + 7) / 8 * 8;  // one item per block, a multiple of the 8 XCDs
+  const size_t lds = (size_t)(cg.L + tc * rd::padded_len(cg.L) + tc * cg.L) * sizeof(cx) +
+                     (size_t)cg.L * sizeof(double);
+  hipLaunchKernelGGL(rd::k_col_b<E>, dim3((unsigned)g), dim3(tc * T), lds, b->stream,
+                     b->d_inst + cl.off, (int)cl.members.size(), per, c->p0, cg, 1, fpw, check);
+}
+
+template <int E>
+void launch_rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_kind, int dst_slot,
+                   int err, rd::StageCoef nx, int G, int check) {
+  const rdcu_ctx *c = cl.c0;
+  const int T = c->L / E;
+  rd::ErrCoef ec{};
+  for (int j = 0; j < 7; ++j) ec.e[j] = EE[j];
+  const size_t lds = rowi_lds(cl, G);
+  for (const BatchGroup &gr : cl.groups) {
+    const rd::Inst *in = b->d_inst + cl.off + gr.i0;
+    int cnt = gr.i1 - gr.i0, rows = (int)c->rows;
+    const unsigned grid = (unsigned)((long)cnt * rows);  // one row per block
+    if (gr.fn) {
+      rd::FftPlan px = c->px;
+      const rd::cx *tw = c->twN;
+      int a_src = src_kind, a_dk = dst_kind, a_ds = dst_slot, a_err = err, a_G = G, a_chk = check;
+      void *args[] = {&in, &cnt, &rows, &px, &tw, &a_src, &a_dk, &a_ds, &a_err, &ec, &nx, &a_G,
+                      &a_chk};
+      (void)hipModuleLaunchKernel(gr.fn, grid, 1, 1, (unsigned)(G * T), 1, 1, (unsigned)lds,
+                                  b->stream, args, nullptr);
+    } else {
+      hipLaunchKernelGGL(rd::k_rowi_b<E>, dim3(grid), dim3(G * T), lds, b->stream, in, cnt, rows,
+                         c->px, c->twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check);
+    }
+  }
+}
+
+void rowf_b(rdcu_batch *b, const BatchClass &cl, int src_kind, rd::StageCoef sc, int rpb,
+            long grid, int check) {
+  if (cl.Ex == 8) launch_rowf_b<8>(b, cl, src_kind, sc, rpb, grid, check);
+  else launch_rowf_b<24>(b, cl, src_kind, sc, rpb, grid, check);
+}
+void col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, long grid, int check) {
+  if (cl.c0->ndim < 2) return;
+  if (cl.E0 == 8) launch_col_b<8>(b, cl, tc, fpw, grid, check);
+  else launch_col_b<24>(b, cl, tc, fpw, grid, check);
+}
+void rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_kind, int dst_slot, int err,
+            rd::StageCoef nx, int G, int check) {
+  if (cl.Ex == 8) launch_rowi_b<8>(b, cl, src_kind, dst_kind, dst_slot, err, nx, G, check);
+  else launch_rowi_b<24>(b, cl, src_kind, dst_kind, dst_slot, err, nx, G, check);
+}
+
+// One plain RHS of every instance of a class (input y, output k slot `slot`
+// or the FSAL slot with DST_K1).
+void rhs_b(rdcu_batch *b, const BatchClass &cl, int dst_kind, int slot, int check) {
+  rd::StageCoef none{};
+  rowf_b(b, cl, rd::SRC_Y, none, cl.rpb, cl.grid_rf, check);
+  col_b(b, cl, cl.tc, cl.fpw, cl.grid_col, check);
+  rowi_b(b, cl, rd::SRC_Y, dst_kind, slot, 0, none, cl.G, check);
+}
+
+// One DP45 attempt of every active instance (the single-instance sequence of
+// attempt_launches_reg per class), then step control for all.
+int enqueue_attempt_b(rdcu_batch *b) {
+  for (const BatchClass &cl : b->classes) {
+    rowf_b(b, cl, rd::SRC_STAGE, stage_coef(2), cl.rpb, cl.grid_rf, 1);
+    for (int s = 2; s <= 7; ++s) {
+      col_b(b, cl, cl.tc, cl.fpw, cl.grid_col, 1);
+      rd::StageCoef nx = s < 7 ? stage_coef(s + 1) : rd::StageCoef{};
+      const int dk = s < 7 ? rd::DST_KS : rd::DST_K7;
+      rowi_b(b, cl, rd::SRC_YT, dk, s - 1, s == 7 && !b->fixed, nx, cl.G, 1);
+    }
+  }
+  hipLaunchKernelGGL(rd::k_control_b, dim3((unsigned)ninst(b)), dim3(256), 0, b->stream, b->d_inst,
+                     b->d_done);
+  HIPCHECK(hipGetLastError());
+  return RDCU_OK;
+}
+
+// Launch shapes per class, timed on the plain RHS of all its instances
+// (every candidate computes the same values).
+int autotune_b(rdcu_batch *b, BatchClass &cl) {
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  auto time_of = [&](const std::function<void()> &f) -> double {
+    f();
+    (void)hipEventRecord(e0, b->stream);
+    const int reps = 5;
+    for (int i = 0; i < reps; ++i) f();
+    (void)hipEventRecord(e1, b->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (hipGetLastError() != hipSuccess) return 1e30;
+    return ms / reps;
+  };
+  const rdcu_ctx *c = cl.c0;
+  const int T = c->L / cl.Ex;
+  rd::StageCoef none{};
+  char line[160];
+  const long caps[] = {0, 4096, 2048};
+  double best = 1e30;
+  for (int rpb = 1; rpb <= 16; rpb *= 2) {
+    if (rpb * T > 256 || (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx) > 160 * 1024)
+      break;
+    for (long cap : caps) {
+      double tm = time_of([&]() { rowf_b(b, cl, rd::SRC_Y, none, rpb, cap, 0); });
+      snprintf(line, sizeof line, "row_fwd rpb=%d grid=%ld %.1fus; ", rpb, cap, tm * 1e3);
+      cl.tune_log += line;
+      if (tm < best) {
+        best = tm;
+        cl.rpb = rpb;
+        cl.grid_rf = cap;
+      }
+    }
+  }
+  if (c->ndim >= 2) {
+    best = 1e30;
+    const int Tc = c->Ns[0] / cl.E0;
+    for (int tc = 1; tc <= 16; tc *= 2) {
+      if (tc * Tc > 256 ||
+          (size_t)(3 * c->Ns[0] + tc * rd::padded_len(c->Ns[0]) + tc * c->Ns[0]) * sizeof(cx) >
+              160 * 1024)
+        break;
+      for (int fpw = cl.max_fpf; fpw >= 1; fpw = fpw > 1 ? (fpw + 1) / 2 : 0) {
+        double tm = time_of([&]() { col_b(b, cl, tc, fpw, 0, 0); });
+        snprintf(line, sizeof line, "col tc=%d fpw=%d %.1fus; ", tc, fpw, tm * 1e3);
+        cl.tune_log += line;
+        if (tm < best) {
+          best = tm;
+          cl.tc = tc;
+          cl.fpw = fpw;
+        }
+        if (fpw == 1) break;
+      }
+    }
+  }
+  best = 1e30;
+  for (int G = c->n; G <= std::max(c->n, cl.nf_max); ++G) {
+    if (G * T > 256 || rowi_lds(cl, G) > 160 * 1024) break;
+    double tm = time_of([&]() { rowi_b(b, cl, rd::SRC_Y, rd::DST_KS, 1, 0, none, G, 0); });
+    snprintf(line, sizeof line, "row_inv G=%d %.1fus; ", G, tm * 1e3);
+    cl.tune_log += line;
+    if (tm < best) {
+      best = tm;
+      cl.G = G;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHECK(hipGetLastError());
+  return RDCU_OK;
+}
+
+// Specialised x-inverse kernels of every (class, dictionary structure) pair,
+// compiled as one hiprtc program: kernel rdcu_rowi_b_<class>_<group>.
+void build_jit_b(rdcu_batch *b) {
+  if (!jit_enabled()) return;
+  std::string src = "#include \"rd_kernels_reg.h\"\n";
+  std::vector<std::string> names;
+  for (size_t ci = 0; ci < b->classes.size(); ++ci) {
+    BatchClass &cl = b->classes[ci];
+    for (size_t gi = 0; gi < cl.groups.size(); ++gi) {
+      const rdcu_ctx *m = b->members[cl.members[cl.groups[gi].i0]].get();
+      const std::string tag = std::to_string(ci) + "_" + std::to_string(gi);
+      src += jit_dict_source(m->n, m->n + m->nf_used, m->h_tfield, m->h_tstart, m->h_fslot,
+                             m->h_fpow, "JitDict_" + tag);
+      src += "extern \"C\" __global__ __launch_bounds__(256) void rdcu_rowi_b_" + tag +
+             "(const rd::Inst *__restrict__ in, int ninst, int rows, rd::FftPlan px, const rd::cx "
+             "*__restrict__ twN, int src_kind, int dst_kind, int dst_slot, int err, rd::ErrCoef "
+             "ec, rd::StageCoef nx, int G, int check) {\n";
+      src += "  rd::rowi_batch<" + std::to_string(cl.Ex) + ", rd::JitDict_" + tag +
+             ">(in, ninst, rows, px, twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check);\n}\n";
+    }
+  }
+  std::vector<char> code;
+  std::string log;
+  if (!jit_compile(src, jit_arch(b->dev), code, log)) {
+    b->jit_log = log;
+    return;
+  }
+  hipModule_t mod = nullptr;
+  if (hipModuleLoadData(&mod, code.data()) != hipSuccess) {
+    b->jit_log = "module load failed";
+    return;
+  }
+  b->jit_mod = mod;
+  for (size_t ci = 0; ci < b->classes.size(); ++ci) {
+    BatchClass &cl = b->classes[ci];
+    for (size_t gi = 0; gi < cl.groups.size(); ++gi) {
+      const std::string nm = "rdcu_rowi_b_" + std::to_string(ci) + "_" + std::to_string(gi);
+      hipFunction_t f = nullptr;
+      if (hipModuleGetFunction(&f, mod, nm.c_str()) == hipSuccess) cl.groups[gi].fn = f;
+    }
+  }
+  b->jit_log = "ok";
+}
+
+std::string structure_key(const rdcu_ctx *m) {
+  std::string k;
+  auto add = [&](const std::vector<int> &v) {
+    for (int x : v) k += std::to_string(x) + ",";
+    k += "|";
+  };
+  add(m->h_tfield);
+  add(m->h_tstart);
+  add(m->h_fslot);
+  add(m->h_fpow);
+  return k;
+}
+
+int read_states_b(rdcu_batch *b) {
+  HIPCHECK(hipMemcpyAsync(b->h_st, b->d_st, sizeof(rd::DevState) * ninst(b),
+                          hipMemcpyDeviceToHost, b->stream));
+  HIPCHECK(hipStreamSynchronize(b->stream));
+  b->nsync++;
+  return RDCU_OK;
+}
+
+int begin_b(rdcu_batch *b, const double *t, const double *h, const double *t1, double hmin,
+            double hmax, int nolog) {
+  const int n = ninst(b);
+  double *ht = b->h_buf, *hh = b->h_buf + n, *ht1 = b->h_buf + 2 * n;
+  for (int i = 0; i < n; ++i) {
+    const int p = b->pos[i];
+    ht[p] = t[i];
+    hh[p] = h[i];
+    ht1[p] = t1 ? t1[i] : INFINITY;
+  }
+  HIPCHECK(hipMemcpyAsync(b->d_t, ht, 3 * sizeof(double) * n, hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(rd::k_begin_b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream,
+                     b->d_inst, n, b->d_t, b->d_h, b->d_t1, hmin, hmax, nolog);
+  HIPCHECK(hipGetLastError());
+  return RDCU_OK;
+}
+
+int ensure_k1_b(rdcu_batch *b, bool force) {
+  bool ok = b->k1_valid && !force;
+  for (auto &m : b->members) ok = ok && m->k1_valid;
+  if (ok) return RDCU_OK;
+  for (const BatchClass &cl : b->classes) rhs_b(b, cl, rd::DST_K1, 0, 0);
+  HIPCHECK(hipGetLastError());
+  for (auto &m : b->members) m->k1_valid = true;
+  b->k1_valid = true;
+  return RDCU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const *y0, double atol,
+                      double rtol, int fixedstep, int device, rdcu_batch **out) {
+  if (!out) return fail(RDCU_EINVAL, "null output handle");
+  *out = nullptr;
+  if (!probs || count < 1) return fail(RDCU_EINVAL, "a batch needs at least one problem");
+  for (int i = 0; i < count; ++i) {
+    if (probs[i].ndim > 2) return fail(RDCU_EUNSUPPORTED, "batched instances are 1-D or 2-D");
+    if (probs[i].n < 1 || probs[i].n > rd::kNMax) return fail(RDCU_EINVAL, "bad field count");
+    long rows = 1;
+    for (int a = 0; a + 1 < probs[i].ndim; ++a) rows *= probs[i].Ns[a];
+    if ((long)probs[i].n * rows > rd::kRedCap)
+      return fail(RDCU_EUNSUPPORTED, "batched instances have at most 4096 grid lines");
+  }
+  std::unique_ptr<rdcu_batch> b(new rdcu_batch());
+  b->dev = device;
+  b->fixed = fixedstep ? 1 : 0;
+  HIPCHECK(hipSetDevice(device));
+  HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  {
+    void *q = nullptr;
+    HIPCHECK(hipMalloc(&q, sizeof(rd::DevState) * count));
+    b->d_st = static_cast<rd::DevState *>(q);
+    HIPCHECK(hipHostMalloc(&q, sizeof(rd::DevState) * count, hipHostMallocDefault));
+    b->h_st = static_cast<rd::DevState *>(q);
+    HIPCHECK(hipMalloc(&q, sizeof(int) * count));
+    b->d_done = static_cast<int *>(q);
+    HIPCHECK(hipHostMalloc(&q, sizeof(int) * count, hipHostMallocDefault));
+    b->h_done = static_cast<int *>(q);
+    HIPCHECK(hipMalloc(&q, 3 * sizeof(double) * count));
+    b->d_t = static_cast<double *>(q);
+    b->d_h = b->d_t + count;
+    b->d_t1 = b->d_t + 2 * count;
+    HIPCHECK(hipHostMalloc(&q, 3 * sizeof(double) * count, hipHostMallocDefault));
+    b->h_buf = static_cast<double *>(q);
+  }
+  // classes: same grid and transform lengths; within a class, instances
+  // with the same dictionary structure are contiguous
+  struct Key {
+    int n, ndim, Ns[3];
+    bool operator<(const Key &o) const {
+      if (n != o.n) return n < o.n;
+      if (ndim != o.ndim) return ndim < o.ndim;
+      for (int a = 0; a < 3; ++a)
+        if (Ns[a] != o.Ns[a]) return Ns[a] < o.Ns[a];
+      return false;
+    }
+  };
+  std::map<Key, std::vector<int>> byclass;
+  for (int i = 0; i < count; ++i) {
+    Key k{probs[i].n, probs[i].ndim, {1, 1, 1}};
+    for (int a = 0; a < probs[i].ndim; ++a) k.Ns[a] = probs[i].Ns[a];
+    byclass[k].push_back(i);
+  }
+  // table order: class by class; members are created in that order so that
+  // their device states are contiguous in table order
+  std::vector<int> order;
+  b->pos.assign(count, 0);
+  b->members.resize(count);
+  int st = RDCU_OK;
+  for (auto &kv : byclass) {
+    // structure groups need the members' dictionaries: create first, sort after
+    for (int i : kv.second) order.push_back(i);
+  }
+  for (int p = 0; p < count; ++p) {
+    const int i = order[p];
+    MemberOpts mo{b->stream, b->d_st + p, b->h_st + p};
+    std::unique_ptr<rdcu_ctx> m;
+    if ((st = create_ctx(&probs[i], atol, rtol, fixedstep, y0 ? y0[i] : nullptr, device, &mo, m)))
+      return fail(st, "instance " + std::to_string(i) + ": " + g_err);
+    b->members[i] = std::move(m);
+  }
+  // within each class, order by dictionary structure; device states follow
+  // the members, so rebuild the table order and move the states with it
+  std::vector<int> final_order;
+  int off = 0;
+  for (auto &kv : byclass) {
+    BatchClass cl;
+    std::map<std::string, std::vector<int>> bys;
+    for (int i : kv.second) bys[structure_key(b->members[i].get())].push_back(i);
+    cl.off = off;
+    for (auto &sv : bys) {
+      BatchGroup gr;
+      gr.i0 = (int)cl.members.size();
+      for (int i : sv.second) cl.members.push_back(i);
+      gr.i1 = (int)cl.members.size();
+      cl.groups.push_back(gr);
+    }
+    for (int i : cl.members) final_order.push_back(i);
+    off += (int)cl.members.size();
+    b->classes.push_back(std::move(cl));
+  }
+  // device state slot of member i must be its table position: reassign
+  {
+    std::vector<rd::DevState> hs(count);
+    for (int p = 0; p < count; ++p) {
+      const int i = final_order[p];
+      hs[p] = *b->members[i]->sh;
+    }
+    for (int p = 0; p < count; ++p) {
+      const int i = final_order[p];
+      b->pos[i] = p;
+      b->members[i]->st = b->d_st + p;
+      b->members[i]->sh = b->h_st + p;
+      b->h_st[p] = hs[p];
+    }
+    HIPCHECK(hipMemcpy(b->d_st, b->h_st, sizeof(rd::DevState) * count, hipMemcpyHostToDevice));
+  }
+  // instance table
+  {
+    std::vector<rd::Inst> tab(count);
+    for (int p = 0; p < count; ++p) {
+      rdcu_ctx *m = b->members[final_order[p]].get();
+      tab[p] = rd::Inst{m->grid, m->bufs, m->st, m->fl_used, m->dictp};
+    }
+    void *q = nullptr;
+    HIPCHECK(hipMalloc(&q, sizeof(rd::Inst) * count));
+    b->d_inst = static_cast<rd::Inst *>(q);
+    HIPCHECK(hipMemcpy(b->d_inst, tab.data(), sizeof(rd::Inst) * count, hipMemcpyHostToDevice));
+  }
+  {
+    static bool lds_raised = false;
+    if (!lds_raised) {
+      const int lim = 160 * 1024;
+      for (const void *f : {(const void *)rd::k_rowf_b<8>, (const void *)rd::k_rowf_b<24>,
+                            (const void *)rd::k_col_b<8>, (const void *)rd::k_col_b<24>,
+                            (const void *)rd::k_rowi_b<8>, (const void *)rd::k_rowi_b<24>})
+        HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+      if ((st = allow_big_lds())) return st;
+      lds_raised = true;
+    }
+  }
+  for (BatchClass &cl : b->classes) {
+    cl.c0 = b->members[cl.members[0]].get();
+    cl.Ex = cl.c0->Ex;
+    cl.E0 = cl.c0->E0;
+    cl.rows = (int)cl.c0->rows;
+    for (int i : cl.members) {
+      rdcu_ctx *m = b->members[i].get();
+      cl.nf_max = std::max(cl.nf_max, m->nf_used);
+      cl.nwords_max = std::max(cl.nwords_max, m->dictp.nwords);
+      cl.max_fpf = std::max(cl.max_fpf, m->max_fpf);
+    }
+    cl.G = cl.c0->n;
+    cl.fpw = cl.max_fpf;
+  }
+  build_jit_b(b.get());
+  for (BatchClass &cl : b->classes) {
+    if (rowi_lds(cl, cl.c0->n) > 160 * 1024)
+      return fail(RDCU_EUNSUPPORTED, "batched instance too large for one block per row");
+    const char *shp = getenv("RDCU_SHAPES");
+    if (!(shp && *shp)) {
+      if ((st = autotune_b(b.get(), cl))) return st;
+    }
+    // members use the class shapes for their own plain-RHS calls
+    for (int i : cl.members) {
+      rdcu_ctx *m = b->members[i].get();
+      m->rf_rpb = cl.rpb;
+      m->cr_tc = cl.tc;
+      m->cr_fpw = cl.fpw;
+      m->ri_G = cl.G;
+    }
+    const char *vb = getenv("RDCU_VERBOSE");
+    if (vb && *vb && *vb != '0')
+      fprintf(stderr, "rdcu batch class %dx%d (%zu instances, %zu structures, jit %s): %s -> "
+                      "%d,%d,%d,%d grids %ld,%ld\n",
+              cl.c0->Ns[0], cl.c0->Ns[1], cl.members.size(), cl.groups.size(),
+              b->jit_log.c_str(), cl.tune_log.c_str(), cl.rpb, cl.tc, cl.fpw, cl.G, cl.grid_rf,
+              cl.grid_col);
+  }
+  HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  st = enqueue_attempt_b(b.get());
+  hipGraph_t graph = nullptr;
+  hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+  if (st) return st;
+  if (ce != hipSuccess) return fail(RDCU_EDEVICE, std::string("capture: ") + hipGetErrorString(ce));
+  b->graph = graph;
+  HIPCHECK(hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0));
+  HIPCHECK(hipStreamSynchronize(b->stream));
+  *out = b.release();
+  return RDCU_OK;
+}
+
+int rdcu_batch_destroy(rdcu_batch *b) {
+  if (b) (void)hipSetDevice(b->dev);
+  delete b;
+  return RDCU_OK;
+}
+
+int rdcu_batch_member(rdcu_batch *b, int i, rdcu_ctx **out) {
+  if (!b || !out) return fail(RDCU_EINVAL, "null argument");
+  if (i < 0 || i >= ninst(b)) return fail(RDCU_EINVAL, "instance index out of range");
+  *out = b->members[i].get();
+  return RDCU_OK;
+}
+
+int rdcu_batch_step(rdcu_batch *b, double *t, double *h, double hmin, double hmax, int *accepted) {
+  (void)hmin;  // as rdcu_step: one attempt regardless of hmin
+  if (!b || !t || !h || !accepted) return fail(RDCU_EINVAL, "null argument");
+  HIPCHECK(hipSetDevice(b->dev));
+  int st;
+  if ((st = begin_b(b, t, h, nullptr, -INFINITY, hmax, 0))) return st;
+  if ((st = ensure_k1_b(b, false))) return st;
+  HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
+  if ((st = read_states_b(b))) return st;
+  for (int i = 0; i < ninst(b); ++i) {
+    const rd::DevState &s = b->h_st[b->pos[i]];
+    if (s.nlog != 1) return fail(RDCU_ESTATE, "attempt of instance " + std::to_string(i) + " did not run");
+    accepted[i] = s.log_acc[0];
+    t[i] = s.t;
+    h[i] = s.h;
+    b->members[i]->nrhs += 6;
+  }
+  return RDCU_OK;
+}
+
+int rdcu_batch_run(rdcu_batch *b, double *t, double *h, double hmin, double hmax, const double *t1,
+                   long long max_attempts, long long *attempts) {
+  if (!b || !t || !h || !t1) return fail(RDCU_EINVAL, "null argument");
+  if (max_attempts < 0) return fail(RDCU_EINVAL, "max_attempts must be >= 0");
+  HIPCHECK(hipSetDevice(b->dev));
+  const int n = ninst(b);
+  std::vector<long long> a0(n);
+  for (int i = 0; i < n; ++i) a0[i] = b->h_st[b->pos[i]].attempts;
+  int st;
+  if ((st = begin_b(b, t, h, t1, hmin, hmax, 1))) return st;
+  // dp45_run computes k1 = f(y) at the start of a run (dp45.cu:157)
+  if ((st = ensure_k1_b(b, true))) return st;
+  long long done = 0;
+  bool all = false;
+  while (!all && done < max_attempts) {
+    const long long k = std::min<long long>(kChunkMax, max_attempts - done);
+    for (long long r = 0; r < k; ++r) HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
+    done += k;
+    HIPCHECK(hipMemcpyAsync(b->h_done, b->d_done, sizeof(int) * n, hipMemcpyDeviceToHost,
+                            b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    b->nsync++;
+    all = true;
+    for (int p = 0; p < n && all; ++p) all = b->h_done[p] != 0;
+  }
+  if ((st = read_states_b(b))) return st;
+  for (int i = 0; i < n; ++i) {
+    const rd::DevState &s = b->h_st[b->pos[i]];
+    t[i] = s.t;
+    h[i] = s.h;
+    const long long na = s.attempts - a0[i];
+    if (attempts) attempts[i] = na;
+    b->members[i]->nrhs += 6 * na;
+  }
+  return RDCU_OK;
+}
+
+}  // extern "C"

@@ -39,7 +39,7 @@ struct DevState {
   int fixed;         // -F
   int nlog;          // attempts logged in this chunk
   int eval_valid;    // last attempt accepted
-  int pad;
+  int nolog;         // batched runs: no attempt log, no halt when it fills
   long long attempts, accepted;
   double log_t[kLogCap], log_h[kLogCap], log_err[kLogCap];
   int log_acc[kLogCap];
@@ -73,6 +73,15 @@ struct Dict {
   const int *tstart;
   const int *fslot;
   const int *fpow;
+};
+
+// Dictionary program of the register path: int32 words, staged into LDS.
+//   words [0, n]: start of field f's term list (word n: end of the program)
+//   term record : nfac, coef low word, coef high word, then nfac (slot, pow)
+// A field's terms keep their input order.  Same meaning as Dict.
+struct DictP {
+  const int *prog;  // device memory
+  int nwords;
 };
 
 // Transformed factors, grouped by field: factors of field f are
@@ -191,6 +200,7 @@ __global__ void k_begin(DevState *st, double t_stop, int set_th, double t, doubl
     st->t_stop = t_stop;
     st->halted = 0;
     st->nlog = 0;
+    st->nolog = 0;
   }
 }
 
@@ -285,6 +295,9 @@ struct ColGeom {
 // read neighbouring columns and the same spectra -- share XCD x's L2.
 __device__ __forceinline__ int xcd_block(int b, int nb) {
   return (b & 7) * (nb >> 3) + (b >> 3);
+}
+__device__ __forceinline__ long xcd_item(long j, long nj) {
+  return (j & 7) * (nj >> 3) + (j >> 3);
 }
 
 __global__ void k_col(Grid g, FftPlan p, ColGeom cg, DevState *st, int check_active, Bufs b,
@@ -522,7 +535,8 @@ __global__ void k_reduce(const double *__restrict__ in, long cnt, double *out, D
 }
 
 // Error norm and step control (dp45.cu:117-146) on the device.
-__global__ void k_control(DevState *st, const double *__restrict__ part, long cnt, long nN) {
+__device__ __forceinline__ void control_body(DevState *st, const double *__restrict__ part,
+                                             long cnt, long nN) {
   __shared__ double v[kRedCap];
   if (!attempt_active(st)) return;
   const int fixed = st->fixed;
@@ -574,6 +588,7 @@ __global__ void k_control(DevState *st, const double *__restrict__ part, long cn
   }
   st->last_norm = norm;
   st->attempts += 1;
+  if (st->nolog) return;
   const int i = st->nlog;
   st->log_t[i] = t;
   st->log_h[i] = h;
@@ -581,6 +596,47 @@ __global__ void k_control(DevState *st, const double *__restrict__ part, long cn
   st->log_acc[i] = acc ? 1 : 0;
   st->nlog = i + 1;
   if (i + 1 >= kLogCap) st->halted = 1;
+}
+
+__global__ void k_control(DevState *st, const double *__restrict__ part, long cnt, long nN) {
+  control_body(st, part, cnt, nN);
+}
+
+// ---------------------------------------------------------------- batches
+// One instance of a batch, as the batched kernels see it (a device table of
+// these; each instance owns its buffers, state, factor list and dictionary).
+struct Inst {
+  Grid g;
+  Bufs b;
+  DevState *st;
+  FacList fl;
+  DictP dict;
+};
+
+// Start of a batched run or step: per-instance t, h, t1; no output stops.
+__global__ void k_begin_b(const Inst *__restrict__ in, int ninst, const double *__restrict__ t,
+                          const double *__restrict__ h, const double *__restrict__ t1,
+                          double hmin, double hmax, int nolog) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst) return;
+  DevState *st = in[i].st;
+  st->t = t[i];
+  st->h = h[i];
+  st->hmin = hmin;
+  st->hmax = hmax;
+  st->t1 = t1[i];
+  st->t_stop = __builtin_inf();
+  st->halted = 0;
+  st->nlog = 0;
+  st->nolog = nolog;
+}
+
+// Step control of every instance (block i = instance i; each instance has at
+// most kRedCap error lines), then done[i] = instance i has finished.
+__global__ void k_control_b(const Inst *__restrict__ in, int *__restrict__ done) {
+  const Inst &I = in[blockIdx.x];
+  control_body(I.st, I.b.part, (long)I.g.n * I.g.rows, (long)I.g.n * I.g.N);
+  if (threadIdx.x == 0) done[blockIdx.x] = attempt_active(I.st) ? 0 : 1;
 }
 
 // Dense output of the last accepted step (dp45.cu:70-76): after the FSAL

@@ -14,15 +14,6 @@
 
 namespace rd {
 
-// Dictionary program of the register path: int32 words, staged into LDS.
-//   words [0, n]: start of field f's term list (word n: end of the program)
-//   term record : nfac, coef low word, coef high word, then nfac (slot, pow)
-// A field's terms keep their input order.  Same meaning as Dict.
-struct DictP {
-  const int *prog;  // device memory
-  int nwords;
-};
-
 // Field f of the dictionary at XM points x0 + stride*i of one row (value rows
 // in LDS, vals[slot*Nx + x]): terms of field f in input order, each a product
 // of integer powers started from 1, accumulated with fma from 0 -- the
@@ -157,17 +148,17 @@ __device__ __forceinline__ cx mult_k(const Grid &g, int blk, const int *nk, cons
 // Block: rpb rows x T threads (T = L/E).  LDS: twiddles L + rpb*padded_len(L)
 // complex.
 template <int E>
-__global__ __launch_bounds__(256) void k_rowf_reg(Grid g, FftPlan px, const cx *__restrict__ twN, DevState *st,
-                           int check_active, Bufs b, int src_kind, const double *src_ptr,
-                           StageCoef sc, int rpb) {
+__device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan &px,
+                                          const cx *__restrict__ twN, DevState *st, const Bufs &b,
+                                          int src_kind, const double *src_ptr, const StageCoef &sc,
+                                          int rpb) {
   extern __shared__ cx smem[];
-  if (check_active && !attempt_active(st)) return;
   const int L = g.L, Nx = g.Nx, Nxh = g.Nxh, T = L / E;
   const long nrows = (long)g.n * g.rows;
   cx *tw = smem, *buf = smem + L;
   const int tid = threadIdx.x;
   const int rr = tid / T, t = tid - rr * T;
-  const long row = (long)blockIdx.x * rpb + rr;
+  const long row = bid * rpb + rr;
   const bool act = row < nrows;
   Prefetch<4, cx> pf_tw;
   pf_tw.load(px.tw, L, tid, blockDim.x);
@@ -238,19 +229,26 @@ __global__ __launch_bounds__(256) void k_rowf_reg(Grid g, FftPlan px, const cx *
   }
 }
 
+template <int E>
+__global__ __launch_bounds__(256) void k_rowf_reg(Grid g, FftPlan px, const cx *__restrict__ twN,
+                                                  DevState *st, int check_active, Bufs b,
+                                                  int src_kind, const double *src_ptr,
+                                                  StageCoef sc, int rpb) {
+  if (check_active && !attempt_active(st)) return;
+  rowf_body<E>(blockIdx.x, g, px, twN, st, b, src_kind, src_ptr, sc, rpb);
+}
+
 // Strided-line pass (modes as k_col).  Block: tc columns x T threads; LDS:
 // twiddles L + tc*padded_len(L) complex (element e of column c at
 // buf[pad8(e)*tc + c]) + tc*L complex holding the forward spectrum (mode 1).
 template <int E>
-__global__ __launch_bounds__(256) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st, int check_active, Bufs b,
-                          FacList fl, int mode, int fpw) {
+__device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan &p,
+                                         const ColGeom &cg, const Bufs &b, const FacList &fl,
+                                         int mode, int fpw) {
   extern __shared__ cx smem[];
-  if (check_active && !attempt_active(st)) return;
   // mode 1: blocks come in groups of nparts, each transforming up to fpw of
   // the field's factors (the forward transform is repeated per part)
   const int nparts = mode == 1 ? cg.nparts : 1;
-  const int lblk = xcd_block(blockIdx.x, gridDim.x);
-  if (lblk >= cg.nblk) return;
   const int part = lblk % nparts;
   const int tile = lblk / nparts;
   const int grp = tile / cg.ntc, ct = tile - grp * cg.ntc;
@@ -329,6 +327,16 @@ __global__ __launch_bounds__(256) void k_col_reg(Grid g, FftPlan p, ColGeom cg, 
   }
 }
 
+template <int E>
+__global__ __launch_bounds__(256) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
+                                                 int check_active, Bufs b, FacList fl, int mode,
+                                                 int fpw) {
+  if (check_active && !attempt_active(st)) return;
+  const int lblk = xcd_block(blockIdx.x, gridDim.x);
+  if (lblk >= cg.nblk) return;
+  col_body<E>(lblk, g, p, cg, b, fl, mode, fpw);
+}
+
 // Inverse x pass for one row position r, all fields: factor lines in groups
 // of G (thread (gl, t) owns line g0+gl), dictionary, stage-7 error tree, and
 // with nx.ns > 0 the fused next stage (stage sum + forward x pass -> S).
@@ -365,16 +373,15 @@ __device__ __forceinline__ void c2r_pre(const Grid &g, const FacList &fl, const 
 }
 
 template <int E, class DE>
-__device__ __forceinline__ void rowi_body(Grid g, FftPlan px, const cx *__restrict__ twN,
-                                          DevState *st, int check_active, Bufs b, FacList fl,
-                                          DictP dict, int src_kind, const double *src_ptr,
-                                          int dst_kind, int dst_slot, double *dst_ptr, int err,
-                                          ErrCoef ec, StageCoef nx, int G) {
+__device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &px,
+                                          const cx *__restrict__ twN, DevState *st, const Bufs &b,
+                                          const FacList &fl, const DictP &dict, int src_kind,
+                                          const double *src_ptr, int dst_kind, int dst_slot,
+                                          double *dst_ptr, int err, const ErrCoef &ec,
+                                          const StageCoef &nx, int G) {
   extern __shared__ cx smem[];
-  if (check_active && !attempt_active(st)) return;
   const int n = g.n, Nx = g.Nx, L = g.L, Nxh = g.Nxh, T = L / E;
   const long N = g.N;
-  const long r = blockIdx.x;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int gl = tid / T, t = tid - gl * T;
   const int nf = fl.nf;
@@ -452,7 +459,7 @@ __device__ __forceinline__ void rowi_body(Grid g, FftPlan px, const cx *__restri
     atol = st->atol;
     rtol = st->rtol;
   }
-  constexpr int XM = 4;
+  constexpr int XM = E <= 8 ? 4 : 2;  // points per thread per pass (register budget)
   for (int xb = 0; xb < Nx; xb += XM * nt) {
     const int x0 = xb + tid;
     for (int f = 0; f < n; ++f) {
@@ -471,7 +478,7 @@ __device__ __forceinline__ void rowi_body(Grid g, FftPlan px, const cx *__restri
           ev[i][5] = yv[idx];
         }
       }
-      DE::template eval<XM>(prog, f, vals, Nx, x0, nt, acc);
+      DE::template eval<XM>(DE::kProg ? prog : dict.prog, f, vals, Nx, x0, nt, acc);
 #pragma unroll
       for (int i = 0; i < XM; ++i) {
         const int x = x0 + nt * i;
@@ -584,8 +591,74 @@ __global__ __launch_bounds__(256) void k_rowi_reg(Grid g, FftPlan px, const cx *
                                                   const double *src_ptr, int dst_kind, int dst_slot,
                                                   double *dst_ptr, int err, ErrCoef ec,
                                                   StageCoef nx, int G) {
-  rowi_body<E, ProgDict>(g, px, twN, st, check_active, b, fl, dict, src_kind, src_ptr, dst_kind,
+  if (check_active && !attempt_active(st)) return;
+  rowi_body<E, ProgDict>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
                          dst_slot, dst_ptr, err, ec, nx, G);
+}
+
+// ---------------------------------------------------------------- batches
+// Persistent launches over the instances of one shape class: the grid walks
+// the (instance, block) items, skipping instances whose attempts have ended;
+// a barrier between items keeps the previous item's LDS reads ahead of the
+// next item's LDS writes.
+
+template <int E>
+__global__ __launch_bounds__(256) void k_rowf_b(const Inst *__restrict__ in, int ninst, int per,
+                                                FftPlan px, const cx *__restrict__ twN,
+                                                int src_kind, StageCoef sc, int rpb, int check) {
+  const long total = (long)ninst * per;
+  bool first = true;
+  for (long it = blockIdx.x; it < total; it += gridDim.x) {
+    const int i = (int)(it / per);
+    DevState *st = in[i].st;
+    if (check && !attempt_active(st)) continue;
+    if (!first) __syncthreads();
+    first = false;
+    rowf_body<E>(it - (long)i * per, in[i].g, px, twN, st, in[i].b, src_kind, nullptr, sc, rpb);
+  }
+}
+
+// One (instance, column tile) item per block (a grid-stride loop around the
+// column pass spills at 24 elements per thread).  The grid is a multiple of
+// 8 and items are ordered so that each XCD takes a contiguous range of
+// logical blocks: neighbouring column tiles of one instance share its L2.
+template <int E>
+__global__ __launch_bounds__(256) void k_col_b(const Inst *__restrict__ in, int ninst, int per,
+                                               FftPlan p, ColGeom cg, int mode, int fpw,
+                                               int check) {
+  const long l = xcd_item(blockIdx.x, gridDim.x);
+  if (l >= (long)ninst * per) return;
+  const int i = (int)(l / per);
+  DevState *st = in[i].st;
+  if (check && !attempt_active(st)) return;
+  col_body<E>((int)(l - (long)i * per), in[i].g, p, cg, in[i].b, in[i].fl, mode, fpw);
+}
+
+// The x-inverse pass takes one (instance, row) item per block: it is the
+// kernel with the most live registers, and a grid-stride loop around it
+// spills at 24 elements per thread.
+template <int E, class DE>
+__device__ __forceinline__ void rowi_batch(const Inst *__restrict__ in, int ninst, int rows,
+                                           const FftPlan &px, const cx *__restrict__ twN,
+                                           int src_kind, int dst_kind, int dst_slot, int err,
+                                           const ErrCoef &ec, const StageCoef &nx, int G,
+                                           int check) {
+  const long it = blockIdx.x;
+  if (it >= (long)ninst * rows) return;
+  const int i = (int)(it / rows);
+  DevState *st = in[i].st;
+  if (check && !attempt_active(st)) return;
+  rowi_body<E, DE>(it - (long)i * rows, in[i].g, px, twN, st, in[i].b, in[i].fl, in[i].dict,
+                   src_kind, nullptr, dst_kind, dst_slot, nullptr, err, ec, nx, G);
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void k_rowi_b(const Inst *__restrict__ in, int ninst, int rows,
+                                                FftPlan px, const cx *__restrict__ twN,
+                                                int src_kind, int dst_kind, int dst_slot, int err,
+                                                ErrCoef ec, StageCoef nx, int G, int check) {
+  rowi_batch<E, ProgDict>(in, ninst, rows, px, twN, src_kind, dst_kind, dst_slot, err, ec, nx,
+                          G, check);
 }
 
 }  // namespace rd

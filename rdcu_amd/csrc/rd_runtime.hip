@@ -154,14 +154,17 @@ struct rdcu_ctx {
   int fixed = 0;
   bool k1_valid = false;
   long long nrhs = 0, nsync = 0;
+  // member of a batch: stream and device state belong to the batch, and
+  // attempts run only through the batch
+  bool member = false;
 
   ~rdcu_ctx() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (jit_mod) (void)hipModuleUnload(jit_mod);
     if (graph) (void)hipGraphDestroy(graph);
     for (void *p : allocs) (void)hipFree(p);
-    if (sh) (void)hipHostFree(sh);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (sh && !member) (void)hipHostFree(sh);
+    if (stream && !member) (void)hipStreamDestroy(stream);
   }
 
   template <class T>
@@ -471,11 +474,15 @@ std::vector<Launch> attempt_launches_reg(rdcu_ctx *c) {
   return v;
 }
 
-// Straight-line evaluator of this problem's dictionary: per field, terms in
+// Straight-line evaluator of a dictionary structure: per field, terms in
 // input order, term = 1 * v^p * ..., acc = fma(C, term, acc) from 0 -- the
-// same operations as dict_field, with slots, powers and coefficients as
-// compile-time constants.
-std::string jit_dict_source(rdcu_ctx *c) {
+// same operations as dict_field, with slots and powers as compile-time
+// constants.  The coefficients are read from the dictionary program at fixed
+// word offsets (uniform loads), so every dictionary with the same structure
+// shares one compiled kernel.
+std::string jit_dict_source(int n, int nslots, const std::vector<int> &tfield,
+                            const std::vector<int> &tstart, const std::vector<int> &fslot,
+                            const std::vector<int> &fpow, const std::string &name = "JitDict") {
   auto powexpr = [](const std::string &v, int p) {
     if (p == 0) return std::string("1.0");
     int a = p < 0 ? -p : p;
@@ -483,77 +490,91 @@ std::string jit_dict_source(rdcu_ctx *c) {
     for (int j = 1; j < a; ++j) e = "(" + e + " * " + v + ")";
     return p < 0 ? "(1.0 / " + e + ")" : e;
   };
+  const int nt = (int)tfield.size();
+  // word offset of each term's coefficient in the program (layout of DictP)
+  std::vector<int> coff(nt, 0);
+  {
+    int pos = n + 1;
+    for (int f = 0; f < n; ++f)
+      for (int i = 0; i < nt; ++i)
+        if (tfield[i] == f) {
+          coff[i] = pos + 1;
+          pos += 3 + 2 * (tstart[i + 1] - tstart[i]);
+        }
+  }
   std::string s;
-  s += "namespace rd {\nstruct JitDict {\n  static constexpr bool kProg = false;\n";
-  s += "  template <int XM>\n  __device__ __forceinline__ static void eval(const int *, int f, "
-       "const double *vals, int Nx, int x0, int stride, double (&acc)[XM]) {\n";
-  s += "#pragma unroll\n    for (int i = 0; i < XM; ++i) {\n";
-  s += "      const int x = x0 + stride * i;\n      const bool in = x < Nx;\n";
-  s += "      double a = 0.0;\n      switch (f) {\n";
+  s += "namespace rd {\nstruct " + name + " {\n  static constexpr bool kProg = false;\n";
+  s += "  template <int XM>\n  __device__ __forceinline__ static void eval(const int *__restrict__ P, "
+       "int f, const double *vals, int Nx, int x0, int stride, double (&acc)[XM]) {\n";
+  s += "    switch (f) {\n";
   char buf[256];
-  const int nt = (int)c->h_tfield.size();
-  for (int f = 0; f < c->n; ++f) {
-    snprintf(buf, sizeof buf, "      case %d: {\n", f);
+  for (int f = 0; f < n; ++f) {
+    snprintf(buf, sizeof buf, "    case %d: {\n", f);
     s += buf;
-    std::vector<char> used(c->n + c->nf_used, 0);
     for (int i = 0; i < nt; ++i)
-      if (c->h_tfield[i] == f)
-        for (int j = c->h_tstart[i]; j < c->h_tstart[i + 1]; ++j) used[c->h_fslot[j]] = 1;
-    for (size_t sl = 0; sl < used.size(); ++sl)
+      if (tfield[i] == f) {
+        snprintf(buf, sizeof buf, "      const double c%d = __hiloint2double(P[%d], P[%d]);\n", i,
+                 coff[i] + 1, coff[i]);
+        s += buf;
+      }
+    s += "#pragma unroll\n      for (int i = 0; i < XM; ++i) {\n";
+    s += "        const int x = x0 + stride * i;\n        const bool in = x < Nx;\n";
+    s += "        double a = 0.0;\n";
+    std::vector<char> used(nslots, 0);
+    for (int i = 0; i < nt; ++i)
+      if (tfield[i] == f)
+        for (int j = tstart[i]; j < tstart[i + 1]; ++j) used[fslot[j]] = 1;
+    for (int sl = 0; sl < nslots; ++sl)
       if (used[sl]) {
-        snprintf(buf, sizeof buf, "        const double v%zu = in ? vals[%zu * Nx + x] : 0.0;\n", sl,
+        snprintf(buf, sizeof buf, "        const double v%d = in ? vals[%d * Nx + x] : 0.0;\n", sl,
                  sl);
         s += buf;
       }
     for (int i = 0; i < nt; ++i) {
-      if (c->h_tfield[i] != f) continue;
+      if (tfield[i] != f) continue;
       s += "        { double t = 1.0;";
-      for (int j = c->h_tstart[i]; j < c->h_tstart[i + 1]; ++j)
-        s += " t = t * " + powexpr("v" + std::to_string(c->h_fslot[j]), c->h_fpow[j]) + ";";
-      snprintf(buf, sizeof buf, " a = __builtin_fma(%a, t, a); }\n", c->h_tcoef[i]);
+      for (int j = tstart[i]; j < tstart[i + 1]; ++j)
+        s += " t = t * " + powexpr("v" + std::to_string(fslot[j]), fpow[j]) + ";";
+      snprintf(buf, sizeof buf, " a = __builtin_fma(c%d, t, a); }\n", i);
       s += buf;
     }
-    s += "      } break;\n";
+    s += "        acc[i] = a;\n      }\n    } break;\n";
   }
-  s += "      default: break;\n      }\n      acc[i] = a;\n    }\n  }\n};\n}  // namespace rd\n";
+  s += "    default:\n#pragma unroll\n      for (int i = 0; i < XM; ++i) acc[i] = 0.0;\n";
+  s += "    }\n  }\n};\n}  // namespace rd\n";
   return s;
 }
 
-// Compiles the x-inverse kernel with the dictionary built in.  On any failure
-// the interpreter kernel (k_rowi_reg) stays in use; RDCU_JIT=0 disables it.
-void build_jit(rdcu_ctx *c) {
-  const char *env = getenv("RDCU_JIT");
-  if (env && !strcmp(env, "0")) return;
-  std::string src = "#include \"rd_kernels_reg.h\"\n" + jit_dict_source(c);
-  src += "extern \"C\" __global__ __launch_bounds__(256) void rdcu_rowi_jit(rd::Grid g, rd::FftPlan px, "
-         "const rd::cx *__restrict__ twN, rd::DevState *st, int check_active, rd::Bufs b, "
-         "rd::FacList fl, rd::DictP dict, int src_kind, const double *src_ptr, int dst_kind, "
-         "int dst_slot, double *dst_ptr, int err, rd::ErrCoef ec, rd::StageCoef nx, int G) {\n";
-  src += "  rd::rowi_body<" + std::to_string(c->Ex) +
-         ", rd::JitDict>(g, px, twN, st, check_active, b, fl, dict, src_kind, src_ptr, dst_kind, "
-         "dst_slot, dst_ptr, err, ec, nx, G);\n}\n";
+std::string jit_arch(int dev) {
   hipDeviceProp_t prop;
   std::string arch = "--offload-arch=gfx950";
-  if (hipGetDeviceProperties(&prop, c->dev) == hipSuccess) {
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
     std::string a = prop.gcnArchName;
     arch = "--offload-arch=" + a.substr(0, a.find(':'));
   }
-  // code objects already compiled in this process, keyed by source and target
+  return arch;
+}
+
+// hiprtc compile of `src` for `arch`; code objects are cached per process,
+// keyed by source and target.  Returns false with a log on failure.
+bool jit_compile(const std::string &src, const std::string &arch, std::vector<char> &code,
+                 std::string &log) {
   static std::mutex cache_mu;
   static std::map<std::string, std::vector<char>> cache;
   const std::string key = arch + "\n" + src;
-  std::vector<char> code;
   {
     std::lock_guard<std::mutex> lk(cache_mu);
     auto it = cache.find(key);
-    if (it != cache.end()) code = it->second;
+    if (it != cache.end()) {
+      code = it->second;
+      return true;
+    }
   }
-  if (code.empty()) {
   hiprtcProgram prog;
   if (hiprtcCreateProgram(&prog, src.c_str(), "rdcu_rowi_jit.hip", kJitNumHeaders, kJitHeaders,
                           kJitHeaderNames) != HIPRTC_SUCCESS) {
-    c->jit_log = "hiprtcCreateProgram failed";
-    return;
+    log = "hiprtcCreateProgram failed";
+    return false;
   }
   const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
                         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
@@ -561,11 +582,11 @@ void build_jit(rdcu_ctx *c) {
   if (r != HIPRTC_SUCCESS) {
     size_t ls = 0;
     hiprtcGetProgramLogSize(prog, &ls);
-    std::string log(ls, '\0');
-    if (ls) hiprtcGetProgramLog(prog, &log[0]);
-    c->jit_log = "hiprtc: " + log;
+    std::string lg(ls, '\0');
+    if (ls) hiprtcGetProgramLog(prog, &lg[0]);
+    log = "hiprtc: " + lg;
     hiprtcDestroyProgram(&prog);
-    return;
+    return false;
   }
   size_t cs = 0;
   hiprtcGetCodeSize(prog, &cs);
@@ -574,7 +595,32 @@ void build_jit(rdcu_ctx *c) {
   hiprtcDestroyProgram(&prog);
   std::lock_guard<std::mutex> lk(cache_mu);
   cache[key] = code;
-  }
+  return true;
+}
+
+bool jit_enabled() {
+  const char *env = getenv("RDCU_JIT");
+  return !(env && !strcmp(env, "0"));
+}
+
+// Compiles the x-inverse kernel with the dictionary structure built in.  On
+// any failure the interpreter kernel (k_rowi_reg) stays in use; RDCU_JIT=0
+// disables it.
+void build_jit(rdcu_ctx *c) {
+  if (!jit_enabled()) return;
+  std::string src = "#include \"rd_kernels_reg.h\"\n" +
+                    jit_dict_source(c->n, c->n + c->nf_used, c->h_tfield, c->h_tstart,
+                                    c->h_fslot, c->h_fpow);
+  src += "extern \"C\" __global__ __launch_bounds__(256) void rdcu_rowi_jit(rd::Grid g, rd::FftPlan px, "
+         "const rd::cx *__restrict__ twN, rd::DevState *st, int check_active, rd::Bufs b, "
+         "rd::FacList fl, rd::DictP dict, int src_kind, const double *src_ptr, int dst_kind, "
+         "int dst_slot, double *dst_ptr, int err, rd::ErrCoef ec, rd::StageCoef nx, int G) {\n";
+  src += "  if (check_active && !rd::attempt_active(st)) return;\n";
+  src += "  rd::rowi_body<" + std::to_string(c->Ex) +
+         ", rd::JitDict>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, "
+         "dst_slot, dst_ptr, err, ec, nx, G);\n}\n";
+  std::vector<char> code;
+  if (!jit_compile(src, jit_arch(c->dev), code, c->jit_log)) return;
   if (hipModuleLoadData(&c->jit_mod, code.data()) != hipSuccess ||
       hipModuleGetFunction(&c->jit_rowi, c->jit_mod, "rdcu_rowi_jit") != hipSuccess) {
     c->jit_log = "module load failed";
@@ -854,15 +900,35 @@ extern "C" {
 const char *rdcu_last_error(void) { return g_err.c_str(); }
 const char *rdcu_backend(void) { return "hip-gfx950"; }
 
+// Where a batch member's stream and device state live (owned by the batch).
+struct MemberOpts {
+  hipStream_t stream;
+  rd::DevState *st;  // device
+  rd::DevState *sh;  // pinned host shadow
+};
+
+int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, const double *y0,
+               int device, const MemberOpts *mo, std::unique_ptr<rdcu_ctx> &c);
+
 int rdcu_create(const rdcu_problem *p, double atol, double rtol, int fixedstep, const double *y0,
                 int device, rdcu_ctx **out) {
   if (!out) return fail(RDCU_EINVAL, "null output handle");
   *out = nullptr;
+  std::unique_ptr<rdcu_ctx> c;
+  int st = create_ctx(p, atol, rtol, fixedstep, y0, device, nullptr, c);
+  if (st) return st;
+  *out = c.release();
+  return RDCU_OK;
+}
+
+int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, const double *y0,
+               int device, const MemberOpts *mo, std::unique_ptr<rdcu_ctx> &c) {
   if (!p) return fail(RDCU_EINVAL, "null problem");
   if (p->n < 1) return fail(RDCU_EINVAL, "n must be >= 1");
   if (p->n > rd::kNMax) return fail(RDCU_EUNSUPPORTED, "at most 8 fields are supported");
   if (p->ndim < 1 || p->ndim > 3) return fail(RDCU_EINVAL, "ndim must be 1, 2 or 3");
-  std::unique_ptr<rdcu_ctx> c(new rdcu_ctx());
+  c.reset(new rdcu_ctx());
+  c->member = mo != nullptr;
   c->n = p->n;
   c->ndim = p->ndim;
   c->N = 1;
@@ -943,7 +1009,8 @@ int rdcu_create(const rdcu_problem *p, double atol, double rtol, int fixedstep, 
   int st = RDCU_OK;
   HIPCHECK(hipSetDevice(device));
   c->dev = device;
-  HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if (mo) c->stream = mo->stream;
+  else HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
   // transform plans and tables
   if ((st = c->make_fft(c->px, c->L))) return st;
@@ -1068,9 +1135,13 @@ int rdcu_create(const rdcu_problem *p, double atol, double rtol, int fixedstep, 
 
   // device state
   {
-    void *q = nullptr;
-    HIPCHECK(hipHostMalloc(&q, sizeof(rd::DevState), hipHostMallocDefault));
-    c->sh = static_cast<rd::DevState *>(q);
+    if (mo) {
+      c->sh = mo->sh;
+    } else {
+      void *q = nullptr;
+      HIPCHECK(hipHostMalloc(&q, sizeof(rd::DevState), hipHostMallocDefault));
+      c->sh = static_cast<rd::DevState *>(q);
+    }
     std::memset(c->sh, 0, sizeof(rd::DevState));
     c->sh->iy = 0;
     c->sh->iyt = 1;
@@ -1085,8 +1156,16 @@ int rdcu_create(const rdcu_problem *p, double atol, double rtol, int fixedstep, 
     c->sh->hmin = 0;
     c->sh->t_stop = INFINITY;
     c->sh->halted = 1;
-    if ((st = c->dalloc(&c->st, 1))) return st;
+    if (mo) c->st = mo->st;
+    else if ((st = c->dalloc(&c->st, 1))) return st;
     HIPCHECK(hipMemcpy(c->st, c->sh, sizeof(rd::DevState), hipMemcpyHostToDevice));
+  }
+
+  if (mo) {  // batch member: the batch tunes, compiles and captures for its classes
+    if (!c->reg)
+      return fail(RDCU_EUNSUPPORTED,
+                  "batched instances need every transform length of the form 2^k or 3*2^k");
+    return RDCU_OK;
   }
 
   if (c->reg) {
@@ -1118,7 +1197,6 @@ int rdcu_create(const rdcu_problem *p, double atol, double rtol, int fixedstep, 
   c->graph = graph;
   HIPCHECK(hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0));
   c->nrhs = 0;
-  *out = c.release();
   return RDCU_OK;
 }
 
@@ -1134,6 +1212,7 @@ int rdcu_destroy(rdcu_ctx *c) {
 int rdcu_step(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, int *accepted) {
   (void)hmin;  // dp45_step does not test hmin; only the run loop does
   if (!c || !t || !h || !accepted) return fail(RDCU_EINVAL, "null argument");
+  if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
   HIPCHECK(hipSetDevice(c->dev));
   int st;
   if ((st = begin(c, INFINITY, 1, *t, *h, -INFINITY, hmax, INFINITY))) return st;
@@ -1151,6 +1230,7 @@ int rdcu_step(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, int *
 int rdcu_run(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double t1, double stop,
              rdcu_step_fn fn, void *user) {
   if (!c || !t || !h) return fail(RDCU_EINVAL, "null argument");
+  if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
   HIPCHECK(hipSetDevice(c->dev));
   int st;
   if ((st = begin(c, stop, 1, *t, *h, hmin, hmax, t1))) return st;
@@ -1190,6 +1270,7 @@ int rdcu_run(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double
 int rdcu_advance(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double t1,
                  long long max_attempts, long long *done) {
   if (!c || !t || !h) return fail(RDCU_EINVAL, "null argument");
+  if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
   HIPCHECK(hipSetDevice(c->dev));
   int st;
   if ((st = begin(c, INFINITY, 1, *t, *h, hmin, hmax, t1))) return st;
@@ -1335,3 +1416,5 @@ int rdcu_get_stats(rdcu_ctx *c, long long out[4]) {
 }
 
 }  // extern "C"
+
+#include "rd_batch.hip"

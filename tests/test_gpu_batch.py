@@ -1,0 +1,63 @@
+"""GPU batches: every instance of a batch (mixed grids, dictionaries and step
+histories in one graph) follows the single-instance arithmetic, so it equals
+the oracle bit for bit, and equals the same problem run alone on the GPU."""
+import numpy as np
+import pytest
+
+from rdcu_amd import Batch, Integrator, Problem, cgle_terms, fhn_terms, glibc_rand_ic
+from rdcu_amd.sweep import benchmark_manifest
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_set():
+    return [Problem(2, [64, 64], [50.0, 50.0], cgle_terms(2, 1.5, 0.4)),
+            Problem(2, [96, 96], [80.0, 80.0], fhn_terms(2, 3.0, 0.05, 1.0)),
+            Problem(2, [64, 64], [40.0, 40.0], fhn_terms(2, 6.0, 0.08, 0.7)),
+            Problem(2, [128, 64], [90.0, 45.0], cgle_terms(2, 2.5, -0.3)),
+            Problem(2, [96, 96], [60.0, 60.0], cgle_terms(2, 0.7, 1.1)),
+            Problem(2, [256], [60.0], cgle_terms(1, 2.0, 0.8))]
+
+
+def test_batch_steps_bitwise_vs_oracle(hlib, olib):
+    probs = mixed_set()
+    ys = [glibc_rand_ic(2, p.N, seed=1000 + i, A=1.0) for i, p in enumerate(probs)]
+    n = len(probs)
+    print("case batch steps", flush=True)
+    with Batch(hlib, probs, ys) as g, Batch(olib, probs, ys) as o:
+        tg, hg = np.zeros(n), np.full(n, 0.01)
+        to, ho = tg.copy(), hg.copy()
+        for k in range(12):
+            tg, hg, ag = g.step(tg, hg)
+            to, ho, ao = o.step(to, ho)
+            assert np.array_equal(tg, to) and np.array_equal(hg, ho), k
+            assert np.array_equal(ag, ao), k
+        for i in range(n):
+            assert np.array_equal(g.member(i).state(), o.member(i).state()), i
+        # run to different end times per instance
+        t1 = np.linspace(0.3, 0.6, n)
+        tg, hg, ng = g.run(tg, hg, t1)
+        to, ho, no = o.run(to, ho, t1)
+        assert np.array_equal(tg, to) and np.array_equal(hg, ho)
+        assert np.array_equal(ng, no)
+        for i in range(n):
+            assert np.array_equal(g.member(i).state(), o.member(i).state()), i
+            mg, mo = g.member(i), o.member(i)
+            f = mg.rhs(mg.state())
+            assert np.array_equal(f, mo.rhs(mo.state())), i
+
+
+def test_batch_equals_single_gpu_runs(hlib):
+    """The sweep manifest's first instances: batched run == each alone."""
+    insts = benchmark_manifest(512, seed=0)[::40]  # 13 instances, both kinds, mixed grids
+    n = len(insts)
+    t1 = 0.2
+    print("case batch vs single", n, flush=True)
+    with Batch(hlib, [x.prob for x in insts], [x.y0() for x in insts]) as b:
+        tb, hb, nb = b.run(np.zeros(n), np.full(n, 0.01), t1)
+        yb = [b.member(i).state() for i in range(n)]
+    for i, x in enumerate(insts):
+        with Integrator(hlib, x.prob, x.y0()) as it:
+            t, h, na = it.advance(0.0, 0.01, t1, max_attempts=1 << 40)
+            assert (t, h, na) == (tb[i], hb[i], nb[i]), i
+            assert np.array_equal(it.state(), yb[i]), i
