@@ -92,6 +92,69 @@ def copy_calibration(mib, reps):
                       "ms": ms, "GBps": 2 * n * 8 / (ms * 1e-3) / 1e9}), flush=True)
 
 
+def sweep_leg(lib, count, t_end, every):
+    """Batched sweep against one-at-a-time plain solves on the same GPU.
+
+    The manifest (rdcu_amd.sweep.benchmark_manifest, PCG64 seed 0): half CGLE,
+    half FitzHugh-Nagumo, grids from {64..256}^2, L in [50, 200], IC seeds
+    1000+i.  Every instance runs from t=0 to t_end at rtol=atol=1e-8.
+    Batched: all instances in one Batch, one rdcu_batch_run.  Serial: every
+    `every`-th instance alone in a plain Integrator (rdcu_advance), the
+    subset's solves/s standing for the whole sweep's; each of those must
+    reproduce its batched t, h, attempt count and final state bit for bit.
+    Setup (context creation, kernel compilation, tuning) is timed apart."""
+    import numpy as np
+    from rdcu_amd import Integrator
+    from rdcu_amd.sweep import benchmark_manifest, Batch
+    insts = benchmark_manifest(count, seed=0)
+    y0s = [x.y0() for x in insts]
+    c0 = time.perf_counter()
+    b = Batch(lib, [x.prob for x in insts], y0s)
+    c1 = time.perf_counter()
+    n = len(insts)
+    t, h, na = b.run(np.zeros(n), np.full(n, 0.01), t_end)  # returns after a stream sync
+    c2 = time.perf_counter()
+    sub = list(range(0, n, every))
+    yb = {i: b.member(i).state() for i in sub}
+    b.close()
+    log(f"batched: {n} instances, setup {c1 - c0:.2f} s, run {c2 - c1:.3f} s, "
+        f"attempts {int(na.sum())} (max {int(na.max())})")
+    # warm the plain path's per-process caches (kernel compilation per
+    # dictionary structure and transform length) before timing
+    warm = {}
+    for i in sub:
+        key = (insts[i].kind, insts[i].grid % 3 == 0)
+        warm.setdefault(key, i)
+    for i in warm.values():
+        Integrator(lib, insts[i].prob, y0s[i]).close()
+    setup_s = run_s = 0.0
+    same = True
+    for i in sub:
+        s0 = time.perf_counter()
+        it = Integrator(lib, insts[i].prob, y0s[i])
+        s1 = time.perf_counter()
+        ts, hs, ns = it.advance(0.0, 0.01, t_end, max_attempts=1 << 40)
+        s2 = time.perf_counter()
+        ys = it.state()
+        it.close()
+        setup_s += s1 - s0
+        run_s += s2 - s1
+        same = same and (ts, hs, ns) == (t[i], h[i], na[i]) and np.array_equal(ys, yb[i])
+    batched = n / (c2 - c1)
+    serial = len(sub) / run_s
+    out = {
+        "instances": n, "t_end": t_end, "manifest": "rdcu_amd.sweep.benchmark_manifest(seed=0)",
+        "batched_solves_per_s": batched, "serial_solves_per_s": serial,
+        "speedup": batched / serial,
+        "batched_solves_per_s_with_setup": n / (c2 - c0),
+        "serial_solves_per_s_with_setup": len(sub) / (setup_s + run_s),
+        "batched_attempts": int(na.sum()), "serial_subset": len(sub),
+        "identical_step_sequences": bool(same),
+    }
+    log("batched:", json.dumps(out))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +175,12 @@ def main():
                          "target for profilers); copy: a plain device copy of --copy-mib MiB, "
                          "the calibration for the HBM byte counters")
     ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--sweep", type=int, default=512,
+                    help="instances of the batched sweep (0 to skip it)")
+    ap.add_argument("--sweep-t", type=float, default=0.5,
+                    help="end time of every sweep instance")
+    ap.add_argument("--serial-every", type=int, default=16,
+                    help="serial baseline on every k-th sweep instance")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each instance lives "
@@ -221,6 +290,8 @@ def main():
         log("cpu:", json.dumps(result["cpu_baseline"]))
 
     it.close()
+    if args.sweep > 0 and nd == 2:
+        result["batched"] = sweep_leg(lib, args.sweep, args.sweep_t, args.serial_every)
     print(json.dumps(result), flush=True)
 
 

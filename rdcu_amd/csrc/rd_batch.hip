@@ -202,9 +202,9 @@ int autotune_b(rdcu_batch *b, BatchClass &cl) {
   const int T = c->L / cl.Ex;
   rd::StageCoef none{};
   char line[160];
-  const long caps[] = {0, 4096, 2048};
+  const long caps[] = {0, 4096};
   double best = 1e30;
-  for (int rpb = 1; rpb <= 16; rpb *= 2) {
+  for (int rpb = std::max(1, 64 / T); rpb <= 128; rpb *= 2) {  // from one wavefront up
     if (rpb * T > 256 || (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx) > 160 * 1024)
       break;
     for (long cap : caps) {
@@ -221,7 +221,7 @@ int autotune_b(rdcu_batch *b, BatchClass &cl) {
   if (c->ndim >= 2) {
     best = 1e30;
     const int Tc = c->Ns[0] / cl.E0;
-    for (int tc = 1; tc <= 16; tc *= 2) {
+    for (int tc = std::max(1, 64 / Tc); tc <= 64; tc *= 2) {
       if (tc * Tc > 256 ||
           (size_t)(3 * c->Ns[0] + tc * rd::padded_len(c->Ns[0]) + tc * c->Ns[0]) * sizeof(cx) >
               160 * 1024)

@@ -688,7 +688,7 @@ int autotune(rdcu_ctx *c) {
   char line[160];
   // x forward: rows per block
   double best = 1e30;
-  for (int rpb = 1; rpb <= 16; rpb *= 2) {
+  for (int rpb = std::max(1, 64 / T); rpb <= 128; rpb *= 2) {  // from one wavefront up
     if (rpb * T > 256 || (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx) > 160 * 1024) break;
     double tm = time_of([&]() { rowf(c, 0, rd::SRC_Y, nullptr, none, rpb); });
     snprintf(line, sizeof line, "row_fwd rpb=%d %.2fus; ", rpb, tm * 1e3);
@@ -708,7 +708,7 @@ int autotune(rdcu_ctx *c) {
       Emin = std::min(c->E0, c->E1);
     }
     int best_fpw = c->max_fpf;
-    for (int tc = 1; tc <= 16; tc *= 2) {
+    for (int tc = std::max(1, 64 / (Lmax / Emin)); tc <= 64; tc *= 2) {
       if (tc * (Lmax / Emin) > 256 ||
           (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) > 160 * 1024)
         break;
