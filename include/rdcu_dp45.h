@@ -154,6 +154,37 @@ int rdcu_batch_run(rdcu_batch *batch, double *t, double *h, double hmin, double 
  * rdcu_advance return RDCU_ESTATE for it). */
 int rdcu_batch_member(rdcu_batch *batch, int i, rdcu_ctx **out);
 
+/* ---- Differentiation: gradients of an output-matching loss ---------------
+ * No counterpart in the reference, whose only derivative code is the Jacobian
+ * chain rule of its stiff path (rdcu.cu:204-216).  These serve fitting
+ * dictionary coefficients to trajectories. */
+
+/* Replaces the dictionary coefficients (coef[nterms], in the problem's term
+ * order); fields, factors and powers stay.  Invalidates the FSAL stage. */
+int rdcu_set_coef(rdcu_ctx *ctx, const double *coef);
+
+/* Step schedule of an adaptive run: from the current state at (t0, h0), runs
+ * attempts as rdcu_advance does while (t < t1 && h > hmin) and writes the size
+ * of every accepted step to hs[0 .. *nsteps).  More than max_steps accepted
+ * steps is an error (RDCU_EINVAL).  The state advances. */
+int rdcu_schedule(rdcu_ctx *ctx, double t0, double h0, double hmin, double hmax, double t1,
+                  long long max_steps, double *hs, long long *nsteps);
+
+/* Loss on a frozen step schedule and its gradient by the discrete adjoint.
+ * From y0[n*N] at t0, takes nsteps DP45 steps of sizes hs[] (every step
+ * accepted, no error control), evaluates the dense output at t_out[0..nout)
+ * (ascending, in (t0, t0 + sum hs]: each in the first step whose end reaches
+ * it, as rdcu_run's stops), and returns
+ *     loss = sum_o sum_x (y(t_o)[x] - target[o*n*N + x])^2 / (nout*n*N).
+ * grad_coef[nterms] and grad_y0[n*N] (either may be NULL; both NULL gives the
+ * loss only) receive dloss/dcoef and dloss/dy0 of exactly these steps: a
+ * reverse sweep over the steps, each recomputing its stages from a checkpoint
+ * of its start state, through the spectral factors and the monomial Jacobian.
+ * The state afterwards is undefined (set it before stepping again). */
+int rdcu_loss_grad(rdcu_ctx *ctx, const double *y0, double t0, long long nsteps,
+                   const double *hs, int nout, const double *t_out, const double *target,
+                   double *loss, double *grad_coef, double *grad_y0);
+
 /* Counters: out[0] attempts, out[1] accepted, out[2] RHS evaluations,
  * out[3] host synchronisations (chunks). */
 int rdcu_get_stats(rdcu_ctx *ctx, long long out[4]);

@@ -767,6 +767,202 @@ int attempt(rdcu_ctx *c, double *t, double *h, double hmax, int *acc) {
 
 }  // namespace
 
+
+// ------------------------------------------------- differentiation (frozen steps)
+// A frozen schedule takes DP45 steps of given sizes, every step accepted with
+// no error control.  Stage inputs Ys[s] and stage derivatives K[s] (s = 0..6
+// for k1..k7) of one step follow attempt() operation for operation, so the
+// trajectory equals a fixed-step run.
+namespace adj {
+
+using Vec = std::vector<double>;
+
+void stages(rdcu_ctx *c, const double *y, double h, Vec *Ys, Vec *K) {
+  const long nN = (long)c->P.n * c->P.N;
+  Ys[0].assign(y, y + nN);
+  K[0].resize(nN);
+  c->E.rhs(Ys[0].data(), K[0].data());
+  for (int s = 2; s <= 7; ++s) {
+    const double *a = STAGE_A[s - 2];
+    Vec &yt = Ys[s - 1];
+    yt.resize(nN);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < nN; ++i) {
+      double sacc = a[0] * K[0][i];
+      for (int j = 1; j < s - 1; ++j)
+        if (a[j] != 0.0) sacc = std::fma(a[j], K[j][i], sacc);
+      yt[i] = std::fma(h, sacc, y[i]);
+    }
+    K[s - 1].resize(nN);
+    c->E.rhs(yt.data(), K[s - 1].data());
+  }
+}
+
+// Dense output of a step (the operations of rdcu_eval).
+void dense(const double *yl, const Vec *K, double x, double hh, double *out, long nN) {
+  const double x2 = x * x, x3 = x2 * x, x4 = x3 * x;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < nN; ++i) {
+    double Q[4];
+    for (int p = 0; p < 4; ++p) {
+      bool first = true;
+      double acc = 0.0;
+      for (int j = 0; j < 7; ++j) {
+        if (PP[j][p] == 0.0) continue;
+        acc = first ? PP[j][p] * K[j][i] : std::fma(PP[j][p], K[j][i], acc);
+        first = false;
+      }
+      Q[p] = acc;
+    }
+    double a = x * Q[0];
+    a = std::fma(x2, Q[1], a);
+    a = std::fma(x3, Q[2], a);
+    a = std::fma(x4, Q[3], a);
+    out[i] = std::fma(hh, a, yl[i]);
+  }
+}
+
+// Values of every dictionary slot at u: slot f < n is field f, slot n+m is
+// referenced factor m (E.used order).
+void slots(rdcu_ctx *c, const double *u, Vec &V) {
+  const Problem &P = c->P;
+  const long N = P.N;
+  const int nf = (int)c->E.used.size();
+  c->E.factors(u, c->E.used);
+  V.resize((size_t)(P.n + nf) * N);
+  std::memcpy(V.data(), u, sizeof(double) * P.n * N);
+  if (nf) std::memcpy(V.data() + (size_t)P.n * N, c->E.R.data(), sizeof(double) * nf * N);
+}
+
+std::vector<int> slot_of_index(const rdcu_ctx *c) {
+  const Problem &P = c->P;
+  std::vector<int> so(P.n * P.nblk, -1);
+  for (int f = 0; f < P.n; ++f) so[f] = f;
+  for (size_t m = 0; m < c->E.used.size(); ++m)
+    so[c->E.used[m].blk * P.n + c->E.used[m].field] = P.n + (int)m;
+  return so;
+}
+
+// D applied to one real field g for derivative block blk (the factor
+// pipeline with g in the place of field `fld`).
+void apply_block(rdcu_ctx *c, const double *g, int fld, int blk, double *out) {
+  const Problem &P = c->P;
+  const long N = P.N;
+  Vec tmp((size_t)P.n * N, 0.0);
+  std::memcpy(tmp.data() + (size_t)fld * N, g, sizeof(double) * N);
+  std::vector<Factor> one{{fld, blk}};
+  c->E.factors(tmp.data(), one);
+  std::memcpy(out, c->E.R.data(), sizeof(double) * N);
+}
+
+// out = J(u)^T w and gc[t] += sum_x w[field_t](x) * term_t(u)(x).  Derivative
+// blocks are symmetric (second derivatives) or antisymmetric (first) as real
+// operators, so D^T g = sign * D g.
+void vjp(rdcu_ctx *c, const double *u, const double *w, double *out, Vec &gc) {
+  const Problem &P = c->P;
+  const long N = P.N;
+  const int n = P.n, nf = (int)c->E.used.size(), ns = n + nf;
+  Vec V;
+  slots(c, u, V);
+  const std::vector<int> so = slot_of_index(c);
+  Vec G((size_t)ns * N, 0.0);
+  const int nt = (int)P.terms.size();
+#pragma omp parallel
+  {
+    Vec gl(nt, 0.0);
+#pragma omp for schedule(static)
+    for (long x = 0; x < N; ++x) {
+      for (int t = 0; t < nt; ++t) {
+        const Term &T = P.terms[t];
+        const double wv = w[(size_t)T.field * N + x];
+        const int nfac = (int)T.idx.size();
+        double v[16];
+        int sl[16];
+        double term = 1.0;
+        for (int j = 0; j < nfac && j < 16; ++j) {
+          sl[j] = so[T.idx[j]];
+          v[j] = V[(size_t)sl[j] * N + x];
+          term = term * ipow(v[j], T.pw[j]);
+        }
+        gl[t] += wv * term;
+        for (int j = 0; j < nfac && j < 16; ++j) {
+          if (T.pw[j] == 0) continue;
+          double d = T.pw[j] * ipow(v[j], T.pw[j] - 1);
+          for (int i = 0; i < nfac; ++i)
+            if (i != j) d = d * ipow(v[i], T.pw[i]);
+          G[(size_t)sl[j] * N + x] += wv * T.coef * d;
+        }
+      }
+    }
+#pragma omp critical
+    for (int t = 0; t < nt; ++t) gc[t] += gl[t];
+  }
+  std::memcpy(out, G.data(), sizeof(double) * n * N);
+  Vec D(N);
+  for (int m = 0; m < nf; ++m) {
+    const Factor &F = c->E.used[m];
+    apply_block(c, &G[(size_t)(n + m) * N], F.field, F.blk, D.data());
+    const double sgn = F.blk <= P.ndim ? -1.0 : 1.0;
+    double *o = out + (size_t)F.field * N;
+    for (long x = 0; x < N; ++x) o[x] += sgn * D[x];
+  }
+}
+
+// Tangent: J(u) du + (df/dcoef) dcoef.
+void jvp(rdcu_ctx *c, const double *u, const double *du, const double *dcoef, double *out) {
+  const Problem &P = c->P;
+  const long N = P.N;
+  const int n = P.n;
+  Vec V, dV;
+  slots(c, u, V);
+  slots(c, du, dV);
+  const std::vector<int> so = slot_of_index(c);
+  const int nt = (int)P.terms.size();
+#pragma omp parallel for schedule(static)
+  for (long x = 0; x < N; ++x) {
+    double acc[64];
+    for (int i = 0; i < n; ++i) acc[i] = 0.0;
+    for (int t = 0; t < nt; ++t) {
+      const Term &T = P.terms[t];
+      const int nfac = (int)T.idx.size();
+      double term = 1.0, dterm = 0.0;
+      for (int j = 0; j < nfac; ++j) {
+        const double vj = V[(size_t)so[T.idx[j]] * N + x];
+        term = term * ipow(vj, T.pw[j]);
+      }
+      for (int j = 0; j < nfac; ++j) {
+        if (T.pw[j] == 0) continue;
+        const double vj = V[(size_t)so[T.idx[j]] * N + x];
+        double d = T.pw[j] * ipow(vj, T.pw[j] - 1) * dV[(size_t)so[T.idx[j]] * N + x];
+        for (int i = 0; i < nfac; ++i)
+          if (i != j) d = d * ipow(V[(size_t)so[T.idx[i]] * N + x], T.pw[i]);
+        dterm += d;
+      }
+      acc[T.field] += T.coef * dterm + (dcoef ? dcoef[t] * term : 0.0);
+    }
+    for (int i = 0; i < n; ++i) out[(size_t)i * N + x] = acc[i];
+  }
+}
+
+// Output placement on the schedule: step index and (x, hh) of each output.
+int place_outputs(double t0, long long ns, const double *hs, int nout, const double *t_out,
+                  std::vector<long long> &ostep, Vec &tn) {
+  tn.assign(ns + 1, t0);
+  for (long long k = 0; k < ns; ++k) tn[k + 1] = tn[k] + hs[k];
+  ostep.assign(nout, -1);
+  long long k = 0;
+  for (int o = 0; o < nout; ++o) {
+    if (o > 0 && t_out[o] < t_out[o - 1]) return fail(RDCU_EINVAL, "output times must ascend");
+    if (!(t_out[o] > t0)) return fail(RDCU_EINVAL, "output time not after t0");
+    while (k < ns && tn[k + 1] < t_out[o]) ++k;
+    if (k >= ns) return fail(RDCU_EINVAL, "output time beyond the schedule");
+    ostep[o] = k;
+  }
+  return RDCU_OK;
+}
+
+}  // namespace adj
+
 extern "C" {
 
 const char *rdcu_last_error(void) { return g_err.c_str(); }
@@ -896,6 +1092,115 @@ int rdcu_batch_run(rdcu_batch *b, double *t, double *h, double hmin, double hmax
     if (st) return st;
     if (attempts) attempts[i] = k;
   }
+  return RDCU_OK;
+}
+
+
+int rdcu_set_coef(rdcu_ctx *c, const double *coef) {
+  if (!c || (!coef && !c->P.terms.empty())) return fail(RDCU_EINVAL, "null argument");
+  for (size_t t = 0; t < c->P.terms.size(); ++t) c->P.terms[t].coef = coef[t];
+  c->k1_valid = false;
+  return RDCU_OK;
+}
+
+int rdcu_schedule(rdcu_ctx *c, double t0, double h0, double hmin, double hmax, double t1,
+                  long long max_steps, double *hs, long long *nsteps) {
+  if (!c || !hs || !nsteps) return fail(RDCU_EINVAL, "null argument");
+  if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
+  double t = t0, h = h0;
+  long long k = 0;
+  c->E.rhs(c->y.data(), c->k[0].data());
+  c->k1_valid = true;
+  while (t < t1 && h > hmin) {
+    const double hu = h;
+    int acc = 0;
+    int st = attempt(c, &t, &h, hmax, &acc);
+    if (st) return st;
+    if (acc) {
+      if (k >= max_steps) return fail(RDCU_EINVAL, "more accepted steps than max_steps");
+      hs[k++] = hu;
+    }
+  }
+  *nsteps = k;
+  return RDCU_OK;
+}
+
+int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const double *hs,
+                   int nout, const double *t_out, const double *target, double *loss,
+                   double *grad_coef, double *grad_y0) {
+  using adj::Vec;
+  if (!c || !y0 || !hs || !t_out || !target || !loss) return fail(RDCU_EINVAL, "null argument");
+  if (ns < 1 || nout < 1) return fail(RDCU_EINVAL, "need at least one step and one output");
+  const long nN = (long)c->P.n * c->P.N;
+  std::vector<long long> ostep;
+  Vec tn;
+  int st = adj::place_outputs(t0, ns, hs, nout, t_out, ostep, tn);
+  if (st) return st;
+  const bool grad = grad_coef || grad_y0;
+  const double cnt = (double)nout * (double)nN;
+  std::vector<Vec> ych;  // start state of every step
+  Vec y(y0, y0 + nN), yo(nN);
+  Vec Ys[7], K[7];
+  Vec gbar((size_t)nout * nN);
+  double L = 0.0;
+  int o = 0;
+  for (long long k = 0; k < ns; ++k) {
+    if (grad) ych.push_back(y);
+    adj::stages(c, y.data(), hs[k], Ys, K);
+    const double hh = tn[k + 1] - tn[k];
+    for (; o < nout && ostep[o] == k; ++o) {
+      adj::dense(y.data(), K, (t_out[o] - tn[k]) / hh, hh, yo.data(), nN);
+      const double *tg = target + (size_t)o * nN;
+      for (long i = 0; i < nN; ++i) {
+        const double d = yo[i] - tg[i];
+        L += d * d;
+        gbar[(size_t)o * nN + i] = 2.0 * d / cnt;
+      }
+    }
+    y = Ys[6];
+  }
+  *loss = L / cnt;
+  if (!grad) return RDCU_OK;
+  // reverse sweep
+  const int nt = (int)c->P.terms.size();
+  Vec gc(nt, 0.0), lam(nN, 0.0), ybar(nN), kbar[7], Yb(nN);
+  o = nout - 1;
+  for (long long k = ns - 1; k >= 0; --k) {
+    const double h = hs[k];
+    adj::stages(c, ych[k].data(), h, Ys, K);
+    const double hh = tn[k + 1] - tn[k];
+    for (int j = 0; j < 7; ++j) kbar[j].assign(nN, 0.0);
+    ybar.assign(nN, 0.0);
+    for (; o >= 0 && ostep[o] == k; --o) {
+      const double x = (t_out[o] - tn[k]) / hh;
+      const double xp[4] = {x, x * x, x * x * x, x * x * x * x};
+      const double *g = &gbar[(size_t)o * nN];
+      for (long i = 0; i < nN; ++i) ybar[i] += g[i];
+      for (int j = 0; j < 7; ++j) {
+        double cj = 0.0;
+        for (int p = 0; p < 4; ++p) cj += PP[j][p] * xp[p];
+        if (cj == 0.0) continue;
+        for (long i = 0; i < nN; ++i) kbar[j][i] += hh * cj * g[i];
+      }
+    }
+    for (int s = 7; s >= 1; --s) {
+      adj::vjp(c, Ys[s - 1].data(), kbar[s - 1].data(), Yb.data(), gc);
+      if (s == 7)
+        for (long i = 0; i < nN; ++i) Yb[i] += lam[i];
+      for (long i = 0; i < nN; ++i) ybar[i] += Yb[i];
+      if (s >= 2) {
+        const double *a = STAGE_A[s - 2];
+        for (int j = 0; j < s - 1; ++j) {
+          if (a[j] == 0.0) continue;
+          for (long i = 0; i < nN; ++i) kbar[j][i] += h * a[j] * Yb[i];
+        }
+      }
+    }
+    lam = ybar;
+  }
+  if (grad_coef)
+    for (int t = 0; t < nt; ++t) grad_coef[t] = gc[t];
+  if (grad_y0) std::memcpy(grad_y0, lam.data(), sizeof(double) * nN);
   return RDCU_OK;
 }
 
@@ -1045,6 +1350,69 @@ void oracle_set_threads(int n) {
   if (n > 0) omp_set_num_threads(n);
 }
 int oracle_get_threads(void) { return omp_get_max_threads(); }
+
+// Tangent-linear of rdcu_loss_grad's loss along (dcoef, dy0) on the same
+// frozen schedule: *dloss = d loss / d eps at eps = 0 for coef + eps*dcoef,
+// y0 + eps*dy0 (forward mode through the same steps).
+int oracle_loss_tl(rdcu_ctx *c, const double *y0, double t0, long long ns, const double *hs,
+                   int nout, const double *t_out, const double *target, const double *dcoef,
+                   const double *dy0, double *loss, double *dloss) {
+  using adj::Vec;
+  const long nN = (long)c->P.n * c->P.N;
+  std::vector<long long> ostep;
+  Vec tn;
+  int st = adj::place_outputs(t0, ns, hs, nout, t_out, ostep, tn);
+  if (st) return st;
+  const double cnt = (double)nout * (double)nN;
+  Vec y(y0, y0 + nN), dy(dy0, dy0 + nN), yo(nN), dyo(nN);
+  Vec Ys[7], K[7], dYs[7], dK[7];
+  double L = 0.0, dL = 0.0;
+  int o = 0;
+  for (long long k = 0; k < ns; ++k) {
+    const double h = hs[k];
+    adj::stages(c, y.data(), h, Ys, K);
+    dYs[0] = dy;
+    dK[0].resize(nN);
+    adj::jvp(c, Ys[0].data(), dYs[0].data(), dcoef, dK[0].data());
+    for (int s = 2; s <= 7; ++s) {
+      const double *a = STAGE_A[s - 2];
+      dYs[s - 1].resize(nN);
+      for (long i = 0; i < nN; ++i) {
+        double sacc = 0.0;
+        for (int j = 0; j < s - 1; ++j) sacc += a[j] * dK[j][i];
+        dYs[s - 1][i] = dy[i] + h * sacc;
+      }
+      dK[s - 1].resize(nN);
+      adj::jvp(c, Ys[s - 1].data(), dYs[s - 1].data(), dcoef, dK[s - 1].data());
+    }
+    const double hh = tn[k + 1] - tn[k];
+    for (; o < nout && ostep[o] == k; ++o) {
+      const double x = (t_out[o] - tn[k]) / hh;
+      adj::dense(y.data(), K, x, hh, yo.data(), nN);
+      const double xp[4] = {x, x * x, x * x * x, x * x * x * x};
+      for (long i = 0; i < nN; ++i) {
+        double q = 0.0;
+        for (int j = 0; j < 7; ++j) {
+          double cj = 0.0;
+          for (int p = 0; p < 4; ++p) cj += PP[j][p] * xp[p];
+          q += cj * dK[j][i];
+        }
+        dyo[i] = dy[i] + hh * q;
+      }
+      const double *tg = target + (size_t)o * nN;
+      for (long i = 0; i < nN; ++i) {
+        const double d = yo[i] - tg[i];
+        L += d * d;
+        dL += 2.0 * d * dyo[i];
+      }
+    }
+    y = Ys[6];
+    dy = dYs[6];
+  }
+  *loss = L / cnt;
+  *dloss = dL / cnt;
+  return RDCU_OK;
+}
 // Real-space factors of u for the given (field, block) list: out[nf*N].
 int oracle_factors(rdcu_ctx *c, const double *u, int nf, const int *field, const int *blk,
                    double *out) {

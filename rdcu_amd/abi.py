@@ -32,7 +32,7 @@ SYMBOLS = (
     "rdcu_eval", "rdcu_get_state", "rdcu_set_state", "rdcu_rhs", "rdcu_get_stats",
     "rdcu_last_error", "rdcu_backend",
     "rdcu_batch_create", "rdcu_batch_destroy", "rdcu_batch_step", "rdcu_batch_run",
-    "rdcu_batch_member",
+    "rdcu_batch_member", "rdcu_set_coef", "rdcu_schedule", "rdcu_loss_grad",
 )
 
 
@@ -92,6 +92,12 @@ def load(path: str) -> ctypes.CDLL:
     lib.rdcu_batch_run.argtypes = [vp, _DP, _DP, ctypes.c_double, ctypes.c_double, _DP,
                                    ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)]
     lib.rdcu_batch_member.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    lib.rdcu_set_coef.argtypes = [vp, _DP]
+    lib.rdcu_schedule.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_longlong, _DP,
+                                  ctypes.POINTER(ctypes.c_longlong)]
+    lib.rdcu_loss_grad.argtypes = [vp, _DP, ctypes.c_double, ctypes.c_longlong, _DP, ctypes.c_int,
+                                   _DP, _DP, _DP, _DP, _DP]
     lib.rdcu_last_error.restype = ctypes.c_char_p
     lib.rdcu_backend.restype = ctypes.c_char_p
     for s in SYMBOLS:
@@ -291,6 +297,41 @@ class Integrator:
         if factors:
             return f, Y.view(np.complex128).reshape(self.prob.nblk, self.prob.n, self.prob.N)
         return f
+
+    def set_coef(self, coef):
+        """Replaces the dictionary coefficients (term order)."""
+        c = np.ascontiguousarray(coef, dtype=np.float64).reshape(-1)
+        if c.size != len(self.prob.terms):
+            raise ValueError(f"{c.size} coefficients for {len(self.prob.terms)} terms")
+        self._check(self.lib.rdcu_set_coef(self.h, _ptr(c)))
+
+    def schedule(self, t0: float, h0: float, t1: float, hmin=0.0, hmax=np.inf,
+                 max_steps=1 << 20) -> np.ndarray:
+        """Accepted step sizes of an adaptive run from the current state."""
+        hs = np.empty(int(max_steps))
+        ns = ctypes.c_longlong()
+        self._check(self.lib.rdcu_schedule(self.h, t0, h0, hmin, hmax, t1, int(max_steps),
+                                           _ptr(hs), ctypes.byref(ns)))
+        return hs[:ns.value].copy()
+
+    def loss_grad(self, y0, t0: float, hs, t_out, target, grad=True):
+        """Loss over outputs on the frozen schedule hs, and (grad=True) its
+        gradients: returns (loss, dloss/dcoef, dloss/dy0)."""
+        y0 = np.ascontiguousarray(y0, dtype=np.float64).reshape(-1)
+        hs = np.ascontiguousarray(hs, dtype=np.float64).reshape(-1)
+        to = np.ascontiguousarray(t_out, dtype=np.float64).reshape(-1)
+        tg = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
+        if y0.size != self.nN or tg.size != to.size * self.nN:
+            raise ValueError("y0 needs n*N values and target nout*n*N")
+        loss = ctypes.c_double()
+        gc = np.empty(max(1, len(self.prob.terms))) if grad else None
+        gy = np.empty(self.nN) if grad else None
+        self._check(self.lib.rdcu_loss_grad(
+            self.h, _ptr(y0), t0, hs.size, _ptr(hs), to.size, _ptr(to), _ptr(tg),
+            ctypes.byref(loss), _ptr(gc) if grad else None, _ptr(gy) if grad else None))
+        if not grad:
+            return loss.value
+        return loss.value, gc[:len(self.prob.terms)], gy
 
     def stats(self):
         out = (ctypes.c_longlong * 4)()

@@ -1,0 +1,214 @@
+// rd_adjoint.h -- device kernels of the discrete adjoint (rdcu_loss_grad).
+//
+// The reverse sweep of a DP45 step needs, per stage s, the vector-Jacobian
+// product J(Y_s)^T w of the dictionary RHS f(u)_i = sum_t C_t prod_j v_j^p_j
+// (v_j the factor values, v = D u for a derivative block D), and the
+// coefficient gradient sum_x w_field(t) * term_t.  With g_k the weight of
+// slot k,
+//     g_k(x)  = sum_t sum_{j: slot_j = k} w_field(t) C_t p_j v_j^(p_j-1) prod_{i!=j} v_i^p_i,
+//     J^T w   = g_u + sum_k D_k^T g_k,      D_k^T = -D_k (first derivatives)
+//                                                  +D_k (second derivatives),
+// the derivative blocks being antisymmetric or symmetric real circulant
+// operators.  k_rowi_vjp computes the factor values of a row (the inverse x
+// pass of the factor spectra, as the RHS does), the slot weights and the
+// coefficient partials; the D_k are then applied by the RHS pipeline of a
+// companion problem, and k_adj_stage accumulates the stage adjoints.
+#pragma once
+#include "rd_kernels_reg.h"
+
+namespace rd {
+
+// Block = G lines of T = L/E threads; one grid row r per block.  LDS: tw (L)
+// and G*padded_len(L) complex, then (n+nf)*Nx values, (n+nf)*Nx weights,
+// n*Nx adjoint inputs, blockDim reduction doubles, the dictionary program.
+// P[tmap[p]*rows + r] += row sum of w*term for the program's p-th term.
+template <int E>
+__global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *__restrict__ twN,
+                                                  Bufs b, FacList fl, DictP dict,
+                                                  const int *__restrict__ tmap,
+                                                  const double *__restrict__ u,
+                                                  const double *__restrict__ W, double *out_id,
+                                                  double *Gk, double *P, int G) {
+  extern __shared__ cx smem[];
+  const int n = g.n, Nx = g.Nx, L = g.L, T = L / E;
+  const long N = g.N, rows = g.rows;
+  const long r = blockIdx.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int gl = tid / T, t = tid - gl * T;
+  const int nf = fl.nf, ns = n + nf;
+  const int PL = padded_len(L);
+  cx *tw = smem, *buf = smem + L;
+  double *vals = reinterpret_cast<double *>(buf + G * PL);
+  double *gacc = vals + (long)ns * Nx;
+  double *wrow = gacc + (long)ns * Nx;
+  double *red = wrow + (long)n * Nx;
+  int *prog = reinterpret_cast<int *>(red + nt);
+  for (int i = tid; i < L; i += nt) tw[i] = px.tw[i];
+  for (int i = tid; i < dict.nwords; i += nt) prog[i] = dict.prog[i];
+  for (int w = tid; w < n * Nx; w += nt) {
+    const long gi = (long)(w / Nx) * N + r * Nx + (w % Nx);
+    vals[w] = u[gi];
+    wrow[w] = W[gi];
+  }
+  for (int w = tid; w < ns * Nx; w += nt) gacc[w] = 0.0;
+  __syncthreads();
+  cx *lb = buf + gl * PL;
+  for (int g0 = 0; g0 < nf; g0 += G) {
+    const int m = g0 + gl;
+    const bool act = gl < G && m < nf;
+    cx v[E];
+    if (act) {
+      c2r_pre<E>(g, fl, b, twN, m, r, t, T, v);
+    } else {
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = cx{0.0, 0.0};
+    }
+    reg_fft<E, +1>(v, lb, 1, tw, px, t, T);
+    if (act) {
+      double *vr = vals + (long)(n + m) * Nx;
+#pragma unroll
+      for (int q = 0; q < E; ++q)
+        *reinterpret_cast<double2 *>(vr + 2 * (t + T * q)) = double2{v[q].re, v[q].im};
+    }
+  }
+  __syncthreads();
+  // terms in program order (fields ascending, input order within a field)
+  for (int f = 0; f < n; ++f) {
+    int p = prog[f];
+    const int pe = prog[f + 1];
+    int pt = 0;  // program-order term index of field f's first term
+    for (int f2 = 0; f2 < f; ++f2)
+      for (int q = prog[f2]; q < prog[f2 + 1]; q += 3 + 2 * prog[q]) ++pt;
+    for (; p < pe; ++pt) {
+      const int nfac = prog[p];
+      const double cc = __hiloint2double(prog[p + 2], prog[p + 1]);
+      const int *fp = prog + p + 3;
+      p += 3 + 2 * nfac;
+      double acc = 0.0;
+      for (int x = tid; x < Nx; x += nt) {
+        const double wv = wrow[f * Nx + x];
+        double term = 1.0;
+        for (int j = 0; j < nfac; ++j) term = term * ipow(vals[fp[2 * j] * Nx + x], fp[2 * j + 1]);
+        acc += wv * term;
+        for (int j = 0; j < nfac; ++j) {
+          const int pj = fp[2 * j + 1];
+          if (pj == 0) continue;
+          const double vj = vals[fp[2 * j] * Nx + x];
+          double d = pj * ipow(vj, pj - 1);
+          for (int i = 0; i < nfac; ++i)
+            if (i != j) d = d * ipow(vals[fp[2 * i] * Nx + x], fp[2 * i + 1]);
+          gacc[fp[2 * j] * Nx + x] += wv * cc * d;
+        }
+      }
+      red[tid] = acc;
+      __syncthreads();
+      int p2 = 1;
+      while (p2 < nt) p2 <<= 1;
+      for (int s = p2 / 2; s > 0; s >>= 1) {  // any block size
+        if (tid < s && tid + s < nt) red[tid] += red[tid + s];
+        __syncthreads();
+      }
+      if (tid == 0) P[(long)tmap[pt] * rows + r] += red[0];
+      __syncthreads();
+    }
+  }
+  for (int w = tid; w < n * Nx; w += nt)
+    out_id[(long)(w / Nx) * N + r * Nx + (w % Nx)] = gacc[w];
+  for (int w = tid; w < nf * Nx; w += nt)
+    Gk[(long)(w / Nx) * N + r * Nx + (w % Nx)] = gacc[n * Nx + w];
+}
+
+// y_n of the current step (the buffer holding y after the FSAL rotation).
+__global__ void k_save_state(const DevState *st, Bufs b, double *dst, long nN) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nN) dst[i] = b.y[st->iy][i];
+}
+
+// Stage input y + h*(sum_j a_j k_j), zero coefficients skipped (the
+// operations of the forward stage sums).
+struct StageIn {
+  const double *k[6];
+  double a[6];
+  int ns;
+};
+__global__ void k_stage_in(const double *__restrict__ y, StageIn si, double h, double *out,
+                           long nN) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nN) return;
+  double sacc = si.a[0] * si.k[0][i];
+  for (int j = 1; j < si.ns; ++j)
+    if (si.a[j] != 0.0) sacc = __builtin_fma(si.a[j], si.k[j][i], sacc);
+  out[i] = __builtin_fma(h, sacc, y[i]);
+}
+
+// Loss terms of one output: gbar = 2 (y - target) / cnt; per-block sums of
+// (y - target)^2 into part[blockIdx].
+__global__ void k_loss(const double *__restrict__ y, const double *__restrict__ tg, double cnt,
+                       double *gbar, double *part, long nN) {
+  __shared__ double s[256];
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (i < nN) {
+    const double d = y[i] - tg[i];
+    d2 = d * d;
+    if (gbar) gbar[i] = 2.0 * d / cnt;
+  }
+  s[threadIdx.x] = d2;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+struct AdjK {
+  double *k[7];
+  double c[7];  // per-array coefficient (0: skip)
+};
+
+// Adjoint of a dense output: ybar += g, kbar_j += c_j g.
+__global__ void k_adj_dense(double *ybar, AdjK kb, const double *__restrict__ g, long nN) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nN) return;
+  const double gv = g[i];
+  ybar[i] += gv;
+  for (int j = 0; j < 7; ++j)
+    if (kb.c[j] != 0.0) kb.k[j][i] += kb.c[j] * gv;
+}
+
+// Adjoint of stage s: Yb = out_id + sum_{m: fmap[m] = field} Dk[m] (+ lam),
+// ybar += Yb, kbar_j += c_j Yb.
+__global__ void k_adj_stage(const double *__restrict__ out_id, const double *__restrict__ Dk,
+                            const int *__restrict__ fmap, int nf, const double *__restrict__ lam,
+                            double *ybar, AdjK kb, long N, long nN) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nN) return;
+  const int f = (int)(i / N);
+  const long x = i - (long)f * N;
+  double Yb = out_id[i];
+  for (int m = 0; m < nf; ++m)
+    if (fmap[m] == f) Yb += Dk[(long)m * N + x];
+  if (lam) Yb += lam[i];
+  ybar[i] += Yb;
+  for (int j = 0; j < 7; ++j)
+    if (kb.c[j] != 0.0) kb.k[j][i] += kb.c[j] * Yb;
+}
+
+}  // namespace rd
+
+namespace rd {
+
+// Switches the integrator between adaptive stepping and a frozen schedule
+// (fixed steps of sizes hs[], no attempt log).
+__global__ void k_set_sched(DevState *st, int fixed, const double *hs, long long ns, int nolog) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->fixed = fixed;
+    st->hsched = hs;
+    st->nsched = ns;
+    st->isched = 0;
+    st->nolog = nolog;
+  }
+}
+
+}  // namespace rd

@@ -1,0 +1,333 @@
+// rd_adjoint.hip -- coefficient updates, step schedules and the discrete
+// adjoint of DP45 on a frozen schedule (included at the end of
+// rd_runtime.hip: same translation unit).
+//
+// Forward: the attempt graph with the device step control switched to fixed
+// steps whose sizes come from the schedule; the start state of every step is
+// saved (checkpoints), and each output is the dense output of the step that
+// reaches it.  Reverse, step by step from the last: the seven stages are
+// recomputed from the checkpoint, then the stage adjoints are accumulated in
+// reverse stage order through vector-Jacobian products of the RHS
+// (k_rowi_vjp + the derivative blocks applied by a companion problem).
+
+namespace {
+
+// RAII list of device allocations of one call.
+struct DevScratch {
+  std::vector<void *> p;
+  ~DevScratch() {
+    for (void *q : p) (void)hipFree(q);
+  }
+  template <class T>
+  int get(T **out, size_t count) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T));
+    if (e != hipSuccess) return fail(RDCU_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    p.push_back(q);
+    *out = static_cast<T *>(q);
+    return RDCU_OK;
+  }
+};
+
+unsigned blocks_of(long n) { return (unsigned)((n + 255) / 256); }
+
+int place_outputs(double t0, long long ns, const double *hs, int nout, const double *t_out,
+                  std::vector<long long> &ostep, std::vector<double> &tn) {
+  tn.assign(ns + 1, t0);
+  for (long long k = 0; k < ns; ++k) tn[k + 1] = tn[k] + hs[k];
+  ostep.assign(nout, -1);
+  long long k = 0;
+  for (int o = 0; o < nout; ++o) {
+    if (o > 0 && t_out[o] < t_out[o - 1]) return fail(RDCU_EINVAL, "output times must ascend");
+    if (!(t_out[o] > t0)) return fail(RDCU_EINVAL, "output time not after t0");
+    while (k < ns && tn[k + 1] < t_out[o]) ++k;
+    if (k >= ns) return fail(RDCU_EINVAL, "output time beyond the schedule");
+    ostep[o] = k;
+  }
+  return RDCU_OK;
+}
+
+// Companion problem: one field per referenced factor m of c, whose RHS is
+// sign_m * D_m applied to it (sign -1 for first derivatives: D^T = -D).
+int adj_companion(rdcu_ctx *c) {
+  if (c->adj_pc) return RDCU_OK;
+  const int nf = c->nf_used;
+  if (nf > rd::kNMax)
+    return fail(RDCU_EUNSUPPORTED, "gradients need at most 8 referenced derivative factors");
+  std::vector<double> coef(nf);
+  std::vector<int> field(nf), nfac(nf, 1), fac(2 * nf);
+  for (int m = 0; m < nf; ++m) {
+    const int blk = c->h_ublk[m];
+    coef[m] = blk <= c->ndim ? -1.0 : 1.0;
+    field[m] = m;
+    fac[2 * m] = blk * nf + m;
+    fac[2 * m + 1] = 1;
+  }
+  rdcu_problem pp{};
+  pp.n = nf;
+  pp.ndim = c->ndim;
+  for (int a = 0; a < 3; ++a) {
+    pp.Ns[a] = c->Ns[a];
+    pp.Ls[a] = c->Ls[a];
+  }
+  pp.nterms = nf;
+  pp.coef = coef.data();
+  pp.field = field.data();
+  pp.nfac = nfac.data();
+  pp.fac = fac.data();
+  std::unique_ptr<rdcu_ctx> pc;
+  int st = create_ctx(&pp, 1e-8, 1e-8, 1, nullptr, c->dev, nullptr, pc);
+  if (st) return st;
+  HIPCHECK(hipStreamSynchronize(pc->stream));
+  HIPCHECK(hipStreamDestroy(pc->stream));
+  pc->stream = c->stream;
+  pc->borrowed_stream = true;
+  c->adj_pc = std::move(pc);
+  return RDCU_OK;
+}
+
+// J(u)^T w -> out_id (identity slots) and Dk (derivative blocks, signed);
+// coefficient partials added into P.
+int vjp(rdcu_ctx *c, const double *u, const double *w, double *out_id, double *Gk, double *Dk,
+        double *P, const int *tmap) {
+  rd::StageCoef none{};
+  rowf(c, 0, rd::SRC_PTR, u, none, c->rf_rpb);
+  for (auto &l : col_launches_reg(c, 0, c->cr_tc)) l.go();
+  const int nf = c->nf_used, T = c->L / c->Ex;
+  const int G = std::max(1, std::min(nf, 256 / T));
+  const int nt = G * T;
+  const size_t lds = (size_t)(c->L + G * rd::padded_len(c->L)) * sizeof(cx) +
+                     (size_t)(2 * (c->n + nf) + c->n) * c->Nx * sizeof(double) +
+                     (size_t)nt * sizeof(double) + (size_t)c->dictp.nwords * sizeof(int);
+  if (lds > 160 * 1024) return fail(RDCU_EUNSUPPORTED, "row too long for the adjoint kernel");
+  if (c->Ex == 8)
+    hipLaunchKernelGGL(rd::k_rowi_vjp<8>, dim3((unsigned)c->rows), dim3(nt), lds, c->stream,
+                       c->grid, c->px, c->twN, c->bufs, c->fl_used, c->dictp, tmap, u, w, out_id,
+                       Gk, P, G);
+  else
+    hipLaunchKernelGGL(rd::k_rowi_vjp<24>, dim3((unsigned)c->rows), dim3(nt), lds, c->stream,
+                       c->grid, c->px, c->twN, c->bufs, c->fl_used, c->dictp, tmap, u, w, out_id,
+                       Gk, P, G);
+  HIPCHECK(hipGetLastError());
+  if (nf > 0) {
+    int st = enqueue_rhs(c->adj_pc.get(), 0, rd::SRC_PTR, Gk, rd::DST_PTR, 0, Dk, 0, 0);
+    if (st) return st;
+  }
+  return RDCU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rdcu_set_coef(rdcu_ctx *c, const double *coef) {
+  if (!c || (!coef && !c->h_tcoef.empty())) return fail(RDCU_EINVAL, "null argument");
+  HIPCHECK(hipSetDevice(c->dev));
+  const int nt = (int)c->h_tcoef.size();
+  for (int i = 0; i < nt; ++i) c->h_tcoef[i] = coef[i];
+  // coefficient words of the program (layout of DictP: fields' term lists)
+  std::vector<int> &prog = c->h_prog;
+  int pos = c->n + 1;
+  for (int f = 0; f < c->n; ++f)
+    for (int i = 0; i < nt; ++i)
+      if (c->h_tfield[i] == f) {
+        std::memcpy(&prog[pos + 1], &c->h_tcoef[i], sizeof(double));
+        pos += 3 + 2 * (c->h_tstart[i + 1] - c->h_tstart[i]);
+      }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  HIPCHECK(hipMemcpy(const_cast<int *>(c->dictp.prog), prog.data(), prog.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  if (nt)
+    HIPCHECK(hipMemcpy(const_cast<double *>(c->dict.tcoef), c->h_tcoef.data(), nt * sizeof(double),
+                       hipMemcpyHostToDevice));
+  c->k1_valid = false;
+  return RDCU_OK;
+}
+
+int rdcu_schedule(rdcu_ctx *c, double t0, double h0, double hmin, double hmax, double t1,
+                  long long max_steps, double *hs, long long *nsteps) {
+  if (!c || !hs || !nsteps) return fail(RDCU_EINVAL, "null argument");
+  if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
+  HIPCHECK(hipSetDevice(c->dev));
+  int st;
+  if ((st = begin(c, INFINITY, 1, t0, h0, hmin, hmax, t1))) return st;
+  c->k1_valid = false;
+  if ((st = ensure_k1(c))) return st;
+  long long k = 0;
+  double tc = t0, hc = h0;
+  while (tc < t1 && hc > hmin) {
+    const long long K = kChunkMax;
+    if ((st = begin(c, INFINITY, 0, 0, 0, hmin, hmax, t1))) return st;
+    for (long long i = 0; i < K; ++i) HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+    if ((st = read_state(c))) return st;
+    const int nl = c->sh->nlog;
+    c->nrhs += 6L * nl;
+    for (int i = 0; i < nl; ++i) {
+      const double hu = i == 0 ? hc : c->sh->log_h[i - 1];  // size the attempt used
+      if (!c->sh->log_acc[i]) continue;
+      if (k >= max_steps) return fail(RDCU_EINVAL, "more accepted steps than max_steps");
+      hs[k++] = hu;
+    }
+    tc = c->sh->t;
+    hc = c->sh->h;
+    if (nl < K) break;
+  }
+  *nsteps = k;
+  return RDCU_OK;
+}
+
+int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const double *hs,
+                   int nout, const double *t_out, const double *target, double *loss,
+                   double *grad_coef, double *grad_y0) {
+  if (!c || !y0 || !hs || !t_out || !target || !loss) return fail(RDCU_EINVAL, "null argument");
+  if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
+  if (ns < 1 || nout < 1) return fail(RDCU_EINVAL, "need at least one step and one output");
+  if (!c->reg) return fail(RDCU_EUNSUPPORTED, "gradients need transform lengths 2^k or 3*2^k");
+  HIPCHECK(hipSetDevice(c->dev));
+  std::vector<long long> ostep;
+  std::vector<double> tn;
+  int st = place_outputs(t0, ns, hs, nout, t_out, ostep, tn);
+  if (st) return st;
+  const bool grad = grad_coef || grad_y0;
+  if (grad && (st = adj_companion(c))) return st;
+  const long N = c->N, nN = (long)c->n * N;
+  const double cnt = (double)nout * (double)nN;
+  const unsigned nbl = blocks_of(nN);
+  DevScratch ds;
+  double *d_hs, *d_tg, *d_part, *ych = nullptr, *gbar = nullptr;
+  if ((st = ds.get(&d_hs, ns)) || (st = ds.get(&d_tg, (size_t)nout * nN)) ||
+      (st = ds.get(&d_part, (size_t)nout * nbl)))
+    return st;
+  if (grad && ((st = ds.get(&ych, (size_t)ns * nN)) || (st = ds.get(&gbar, (size_t)nout * nN))))
+    return st;
+  HIPCHECK(hipMemcpyAsync(d_hs, hs, ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(d_tg, target, (size_t)nout * nN * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+  if ((st = rdcu_set_state(c, y0))) return st;
+
+  // ---- forward on the frozen schedule
+  hipLaunchKernelGGL(rd::k_set_sched, dim3(1), dim3(1), 0, c->stream, c->st, 1, d_hs, ns, 1);
+  if ((st = begin(c, INFINITY, 1, t0, hs[0], -INFINITY, INFINITY, INFINITY))) return st;
+  hipLaunchKernelGGL(rd::k_set_sched, dim3(1), dim3(1), 0, c->stream, c->st, 1, d_hs, ns, 1);
+  c->k1_valid = false;
+  if ((st = ensure_k1(c))) return st;
+  rd::DenseCoef dc;
+  for (int j = 0; j < 7; ++j)
+    for (int q = 0; q < 4; ++q) dc.p[j][q] = PP[j][q];
+  int o = 0;
+  for (long long k = 0; k < ns; ++k) {
+    if (grad)
+      hipLaunchKernelGGL(rd::k_save_state, dim3(blocks_of(nN)), dim3(256), 0, c->stream, c->st,
+                         c->bufs, ych + (size_t)k * nN, nN);
+    HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+    for (; o < nout && ostep[o] == k; ++o) {
+      hipLaunchKernelGGL(rd::k_dense, dim3(blocks_of(nN)), dim3(256), 0, c->stream, c->st, c->bufs,
+                         dc, t_out[o], c->yeval, nN);
+      hipLaunchKernelGGL(rd::k_loss, dim3(nbl), dim3(256), 0, c->stream, c->yeval,
+                         d_tg + (size_t)o * nN, cnt, grad ? gbar + (size_t)o * nN : nullptr,
+                         d_part + (size_t)o * nbl, nN);
+    }
+  }
+  HIPCHECK(hipGetLastError());
+  c->nrhs += 6 * ns;
+  hipLaunchKernelGGL(rd::k_set_sched, dim3(1), dim3(1), 0, c->stream, c->st, c->fixed, nullptr,
+                     0LL, 0);
+  std::vector<double> part((size_t)nout * nbl);
+  HIPCHECK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+  if ((st = read_state(c))) return st;
+  c->k1_valid = false;
+  double L = 0.0;
+  for (double v : part) L += v;
+  *loss = L / cnt;
+  if (!grad) return RDCU_OK;
+
+  // ---- reverse sweep
+  const int nf = c->nf_used, nt = (int)c->h_tcoef.size();
+  double *K[7], *Ys[7], *kbar[7], *ybar, *lam, *out_id, *Gk, *Dk, *P;
+  int *tmap, *fmap;
+  for (int j = 0; j < 7; ++j)
+    if ((st = ds.get(&K[j], nN)) || (st = ds.get(&Ys[j], nN)) || (st = ds.get(&kbar[j], nN)))
+      return st;
+  if ((st = ds.get(&ybar, nN)) || (st = ds.get(&lam, nN)) || (st = ds.get(&out_id, nN)) ||
+      (st = ds.get(&Gk, (size_t)nf * N)) || (st = ds.get(&Dk, (size_t)nf * N)) ||
+      (st = ds.get(&P, (size_t)std::max(1, nt) * c->rows)) || (st = ds.get(&tmap, nt)) ||
+      (st = ds.get(&fmap, nf)))
+    return st;
+  {
+    std::vector<int> tm;
+    for (int f = 0; f < c->n; ++f)
+      for (int i = 0; i < nt; ++i)
+        if (c->h_tfield[i] == f) tm.push_back(i);
+    if (nt) HIPCHECK(hipMemcpyAsync(tmap, tm.data(), nt * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (nf)
+      HIPCHECK(hipMemcpyAsync(fmap, c->h_ufld.data(), nf * sizeof(int), hipMemcpyHostToDevice,
+                              c->stream));
+  }
+  HIPCHECK(hipMemsetAsync(lam, 0, nN * sizeof(double), c->stream));
+  HIPCHECK(hipMemsetAsync(P, 0, (size_t)std::max(1, nt) * c->rows * sizeof(double), c->stream));
+  o = nout - 1;
+  for (long long k = ns - 1; k >= 0; --k) {
+    const double h = hs[k];
+    const double *y = ych + (size_t)k * nN;
+    // stages from the checkpoint (the forward's operations)
+    if ((st = enqueue_rhs(c, 0, rd::SRC_PTR, y, rd::DST_PTR, 0, K[0], 0, 0))) return st;
+    for (int s = 2; s <= 7; ++s) {
+      rd::StageIn si{};
+      si.ns = s - 1;
+      for (int j = 0; j < s - 1; ++j) {
+        si.k[j] = K[j];
+        si.a[j] = STAGE_A[s - 2][j];
+      }
+      hipLaunchKernelGGL(rd::k_stage_in, dim3(blocks_of(nN)), dim3(256), 0, c->stream, y, si, h,
+                         Ys[s - 1], nN);
+      if ((st = enqueue_rhs(c, 0, rd::SRC_PTR, Ys[s - 1], rd::DST_PTR, 0, K[s - 1], 0, 0)))
+        return st;
+    }
+    c->nrhs += 7;
+    for (int j = 0; j < 7; ++j)
+      HIPCHECK(hipMemsetAsync(kbar[j], 0, nN * sizeof(double), c->stream));
+    HIPCHECK(hipMemsetAsync(ybar, 0, nN * sizeof(double), c->stream));
+    const double hh = tn[k + 1] - tn[k];
+    for (; o >= 0 && ostep[o] == k; --o) {
+      const double x = (t_out[o] - tn[k]) / hh;
+      const double xp[4] = {x, x * x, x * x * x, x * x * x * x};
+      rd::AdjK kb{};
+      for (int j = 0; j < 7; ++j) {
+        double cj = 0.0;
+        for (int p = 0; p < 4; ++p) cj += PP[j][p] * xp[p];
+        kb.k[j] = kbar[j];
+        kb.c[j] = hh * cj;
+      }
+      hipLaunchKernelGGL(rd::k_adj_dense, dim3(blocks_of(nN)), dim3(256), 0, c->stream, ybar, kb,
+                         gbar + (size_t)o * nN, nN);
+    }
+    for (int s = 7; s >= 1; --s) {
+      const double *u = s == 1 ? y : Ys[s - 1];
+      if ((st = vjp(c, u, kbar[s - 1], out_id, Gk, Dk, P, tmap))) return st;
+      rd::AdjK kb{};
+      for (int j = 0; j < 7; ++j) kb.k[j] = kbar[j];
+      if (s >= 2)
+        for (int j = 0; j < s - 1; ++j) kb.c[j] = h * STAGE_A[s - 2][j];
+      hipLaunchKernelGGL(rd::k_adj_stage, dim3(blocks_of(nN)), dim3(256), 0, c->stream, out_id, Dk,
+                         fmap, nf, s == 7 ? lam : nullptr, ybar, kb, N, nN);
+    }
+    HIPCHECK(hipGetLastError());
+    std::swap(lam, ybar);
+  }
+  std::vector<double> hp((size_t)std::max(1, nt) * c->rows);
+  HIPCHECK(hipMemcpyAsync(hp.data(), P, hp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+  if (grad_y0)
+    HIPCHECK(hipMemcpyAsync(grad_y0, lam, nN * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (grad_coef)
+    for (int t = 0; t < nt; ++t) {
+      double sacc = 0.0;
+      for (long r = 0; r < c->rows; ++r) sacc += hp[(size_t)t * c->rows + r];
+      grad_coef[t] = sacc;
+    }
+  return RDCU_OK;
+}
+
+}  // extern "C"

@@ -41,6 +41,9 @@ struct DevState {
   int eval_valid;    // last attempt accepted
   int nolog;         // batched runs: no attempt log, no halt when it fills
   long long attempts, accepted;
+  // frozen schedule (fixed steps): step i has size hsched[i]; null otherwise
+  const double *hsched;
+  long long nsched, isched;
   double log_t[kLogCap], log_h[kLogCap], log_err[kLogCap];
   int log_acc[kLogCap];
 };
@@ -567,6 +570,12 @@ __device__ __forceinline__ void control_body(DevState *st, const double *__restr
       h = h * factor;
       if (h > st->hmax) h = st->hmax;
       st->h = h;
+    } else if (st->hsched) {
+      const long long i = ++st->isched;
+      if (i < st->nsched) {
+        h = st->hsched[i];
+        st->h = h;
+      }
     }
     const int iy = st->iy, iyt = st->iyt, iyl = st->iyl;
     st->iy = iyt;

@@ -27,6 +27,7 @@
 
 #include "rd_kernels.h"
 #include "rd_kernels_reg.h"
+#include "rd_adjoint.h"
 #include "rdcu_dp45.h"
 
 #include "rd_jit_src.inc"
@@ -157,14 +158,19 @@ struct rdcu_ctx {
   // member of a batch: stream and device state belong to the batch, and
   // attempts run only through the batch
   bool member = false;
+  bool borrowed_stream = false;  // stream owned by another context
+  std::vector<int> h_ublk, h_ufld, h_prog;  // referenced factors; dictionary program
+  // adjoint: companion problem applying the derivative blocks to slot weights
+  std::unique_ptr<rdcu_ctx> adj_pc;
 
   ~rdcu_ctx() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (jit_mod) (void)hipModuleUnload(jit_mod);
     if (graph) (void)hipGraphDestroy(graph);
     for (void *p : allocs) (void)hipFree(p);
+    adj_pc.reset();
     if (sh && !member) (void)hipHostFree(sh);
-    if (stream && !member) (void)hipStreamDestroy(stream);
+    if (stream && !member && !borrowed_stream) (void)hipStreamDestroy(stream);
   }
 
   template <class T>
@@ -762,6 +768,8 @@ int allow_big_lds() {
   HIPCHECK(hipFuncSetAttribute((const void *)rd::k_col_reg<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_reg<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_reg<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_vjp<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_vjp<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return RDCU_OK;
 }
 
@@ -1088,6 +1096,9 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
         (st = c->upload(&ab, ablk)) || (st = c->upload(&af, afld)) || (st = c->upload(&as, astart)))
       return st;
     c->fl_used = rd::FacList{c->nf_used, ub, uf, us};
+    c->h_ublk = ublk;
+    c->h_ufld = ufld;
+    c->h_prog = prog;
     c->fl_all = rd::FacList{c->nf_all, ab, af, as};
   }
   if ((st = choose_shapes(c.get()))) return st;
@@ -1418,3 +1429,4 @@ int rdcu_get_stats(rdcu_ctx *c, long long out[4]) {
 }  // extern "C"
 
 #include "rd_batch.hip"
+#include "rd_adjoint.hip"

@@ -40,7 +40,23 @@ def olib():
     lib.oracle_set_threads.restype = None
     lib.oracle_factors.argtypes = [ctypes.c_void_p, D, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                    ctypes.POINTER(ctypes.c_int), D]
+    lib.oracle_loss_tl.argtypes = [ctypes.c_void_p, D, ctypes.c_double, ctypes.c_longlong, D,
+                                   ctypes.c_int, D, D, D, D, D, D]
     return lib
+
+
+def loss_tl(olib, it, y0, t0, hs, t_out, target, dcoef, dy0):
+    """Oracle tangent-linear: (loss, d loss along (dcoef, dy0))."""
+    import ctypes
+    from rdcu_amd.abi import _ptr
+    arr = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+           for a in (y0, hs, t_out, target, dcoef, dy0)]
+    L, dL = ctypes.c_double(), ctypes.c_double()
+    rc = olib.oracle_loss_tl(it.h, _ptr(arr[0]), t0, arr[1].size, _ptr(arr[1]), arr[2].size,
+                             _ptr(arr[2]), _ptr(arr[3]), _ptr(arr[4]), _ptr(arr[5]),
+                             ctypes.byref(L), ctypes.byref(dL))
+    assert rc == 0, olib.rdcu_last_error()
+    return L.value, dL.value
 
 
 @pytest.fixture(scope="session")

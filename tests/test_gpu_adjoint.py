@@ -1,0 +1,111 @@
+"""GPU discrete adjoint (rdcu_loss_grad) against the oracle's adjoint and
+tangent-linear runs and against central finite differences, and the frozen
+forward against the adaptive run it replays."""
+import numpy as np
+import pytest
+
+from conftest import loss_tl
+from rdcu_amd import Integrator, Problem, cgle_terms, glibc_rand_ic
+
+pytestmark = pytest.mark.gpu
+
+
+def outputs_of_run(it, t_out, t1):
+    outs, nxt = [], [0]
+
+    def cb(t, h, ok, cur):
+        while ok and nxt[0] < len(t_out) and t >= t_out[nxt[0]]:
+            outs.append(it.eval(t_out[nxt[0]]))
+            nxt[0] += 1
+        return t_out[nxt[0]] if nxt[0] < len(t_out) else np.inf
+
+    it.run(0.0, 0.01, t1, stop=t_out[0], callback=cb)
+    return np.concatenate(outs)
+
+
+def fit_problem(prob, y0, t_out, t1, lib, rtol, noise=0.05, seed=0):
+    """Target from perturbed coefficients; schedule of the unperturbed model."""
+    rng = np.random.default_rng(seed)
+    coef = np.array([t.coef for t in prob.terms])
+    pert = coef * (1 + noise * rng.standard_normal(coef.size))
+    with Integrator(lib, prob, y0, atol=rtol, rtol=rtol) as it:
+        it.set_coef(pert)
+        target = outputs_of_run(it, t_out, t1)
+    it = Integrator(lib, prob, y0, atol=rtol, rtol=rtol)
+    hs = it.schedule(0.0, 0.01, t1)
+    return it, coef, target, hs
+
+
+def test_schedule_and_frozen_forward_bitwise(hlib, olib):
+    prob = Problem(2, [64, 64], [50.0, 50.0], cgle_terms(2, 2.0, 0.8))
+    y0 = glibc_rand_ic(2, prob.N, seed=4, A=1.0)
+    t_out = np.array([0.1, 0.2, 0.3])
+    print("case frozen forward", flush=True)
+    with Integrator(hlib, prob, y0, atol=1e-10, rtol=1e-10) as g, \
+            Integrator(olib, prob, y0, atol=1e-10, rtol=1e-10) as o:
+        outs = outputs_of_run(g, t_out, 0.3)
+        g.set_state(y0)
+        hg = g.schedule(0.0, 0.01, 0.3)
+        ho = o.schedule(0.0, 0.01, 0.3)
+        assert np.array_equal(hg, ho) and hg.size > 10
+        assert g.loss_grad(y0, 0.0, hg, t_out, outs, grad=False) == 0.0
+        tgt = outs * 0.9 + 0.01
+        Lg = g.loss_grad(y0, 0.0, hg, t_out, tgt, grad=False)
+        Lo = o.loss_grad(y0, 0.0, ho, t_out, tgt, grad=False)
+        assert Lg == pytest.approx(Lo, rel=1e-13)
+
+
+def test_gradients_match_oracle_adjoint_and_tangent_linear(hlib, olib):
+    prob = Problem(2, [48, 64], [40.0, 55.0], cgle_terms(2, 2.0, 0.8))
+    y0 = glibc_rand_ic(2, prob.N, seed=5, A=1.0)
+    t_out = np.array([0.1, 0.2, 0.3])
+    print("case gradients vs oracle", flush=True)
+    g, coef, target, hs = fit_problem(prob, y0, t_out, 0.3, hlib, 1e-10)
+    Lg, gcg, gyg = g.loss_grad(y0, 0.0, hs, t_out, target)
+    g.close()
+    with Integrator(olib, prob, y0, atol=1e-10, rtol=1e-10) as o:
+        Lo, gco, gyo = o.loss_grad(y0, 0.0, hs, t_out, target)
+        assert Lg == pytest.approx(Lo, rel=1e-12)
+        assert np.abs(gcg - gco).max() <= 1e-10 * np.abs(gco).max()
+        assert np.abs(gyg - gyo).max() <= 1e-10 * np.abs(gyo).max()
+        rng = np.random.default_rng(3)
+        for _ in range(2):
+            dc, dy = rng.standard_normal(coef.size), rng.standard_normal(y0.size)
+            _, dL = loss_tl(olib, o, y0, 0.0, hs, t_out, target, dc, dy)
+            assert float(gcg @ dc + gyg @ dy) == pytest.approx(dL, rel=1e-8)
+
+
+def test_adjoint_config_vs_central_differences(hlib, olib):
+    """2-D CGLE 256x256, L=100, t=10, 20 outputs, rtol=atol=1e-10, target from
+    coefficients with 5% relative noise: every coefficient gradient against
+    central differences (h=1e-6) on the frozen schedule; the full gradient
+    against the oracle's tangent-linear run over the first half time unit."""
+    prob = Problem(2, [256, 256], [100.0, 100.0], cgle_terms(2, 2.0, 0.8))
+    y0 = glibc_rand_ic(2, prob.N, seed=1, A=1.0)
+    t_out = np.arange(1, 21) * 0.5
+    print("case adjoint 256x256 t=10", flush=True)
+    g, coef, target, hs = fit_problem(prob, y0, t_out, 10.0, hlib, 1e-10)
+    L, gc, gy = g.loss_grad(y0, 0.0, hs, t_out, target)
+    assert L > 0 and np.all(np.isfinite(gc)) and np.all(np.isfinite(gy))
+    eps = 1e-6
+    for t in range(coef.size):
+        cp, cm = coef.copy(), coef.copy()
+        cp[t] += eps
+        cm[t] -= eps
+        g.set_coef(cp)
+        Lp = g.loss_grad(y0, 0.0, hs, t_out, target, grad=False)
+        g.set_coef(cm)
+        Lm = g.loss_grad(y0, 0.0, hs, t_out, target, grad=False)
+        fd = (Lp - Lm) / (2 * eps)
+        assert fd == pytest.approx(gc[t], rel=1e-6, abs=1e-6 * np.abs(gc).max()), (t, fd, gc[t])
+    g.set_coef(coef)
+    # tangent-linear check over the steps up to t = 0.5 (one output)
+    k = int(np.searchsorted(np.cumsum(hs), 0.5)) + 1
+    L1, gc1, gy1 = g.loss_grad(y0, 0.0, hs[:k], t_out[:1], target[:y0.size])
+    g.close()
+    with Integrator(olib, prob, y0, atol=1e-10, rtol=1e-10) as o:
+        rng = np.random.default_rng(9)
+        dc, dy = rng.standard_normal(coef.size), rng.standard_normal(y0.size)
+        Lt, dL = loss_tl(olib, o, y0, 0.0, hs[:k], t_out[:1], target[:y0.size], dc, dy)
+    assert Lt == pytest.approx(L1, rel=1e-12)
+    assert float(gc1 @ dc + gy1 @ dy) == pytest.approx(dL, rel=1e-6)
