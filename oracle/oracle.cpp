@@ -867,9 +867,10 @@ void vjp(rdcu_ctx *c, const double *u, const double *w, double *out, Vec &gc) {
   const std::vector<int> so = slot_of_index(c);
   Vec G((size_t)ns * N, 0.0);
   const int nt = (int)P.terms.size();
+  std::vector<Vec> part(omp_get_max_threads(), Vec(nt, 0.0));
 #pragma omp parallel
   {
-    Vec gl(nt, 0.0);
+    Vec &gl = part[omp_get_thread_num()];
 #pragma omp for schedule(static)
     for (long x = 0; x < N; ++x) {
       for (int t = 0; t < nt; ++t) {
@@ -894,9 +895,9 @@ void vjp(rdcu_ctx *c, const double *u, const double *w, double *out, Vec &gc) {
         }
       }
     }
-#pragma omp critical
-    for (int t = 0; t < nt; ++t) gc[t] += gl[t];
   }
+  for (const Vec &gl : part)  // fixed order: results do not depend on thread timing
+    for (int t = 0; t < nt; ++t) gc[t] += gl[t];
   std::memcpy(out, G.data(), sizeof(double) * n * N);
   Vec D(N);
   for (int m = 0; m < nf; ++m) {
