@@ -155,6 +155,66 @@ def sweep_leg(lib, count, t_end, every):
     return out
 
 
+def adjoint_leg(lib):
+    """Gradient of an output-matching loss by the discrete adjoint.
+
+    2-D CGLE 256x256, L=100, IC amplitude 1 seed 1, rtol=atol=1e-10, outputs
+    at t = 0.5, 1, ..., 10; the target comes from the coefficients with 5%
+    relative Gaussian noise (seed 0), the gradient is taken at the nominal
+    coefficients on the step schedule of their adaptive run.  Reports the
+    time of one loss-only evaluation and of loss + gradient (all 18
+    coefficients and the initial state), and central differences (h=1e-6) on
+    three coefficients against the adjoint."""
+    import numpy as np
+    from rdcu_amd import Integrator, Problem, cgle_terms, glibc_rand_ic
+    prob = Problem(2, [256, 256], [100.0, 100.0], cgle_terms(2, 2.0, 0.8))
+    y0 = glibc_rand_ic(2, prob.N, seed=1, A=1.0)
+    t_out = np.arange(1, 21) * 0.5
+    coef = np.array([t.coef for t in prob.terms])
+    rng = np.random.default_rng(0)
+    pert = coef * (1 + 0.05 * rng.standard_normal(coef.size))
+    with Integrator(lib, prob, y0, atol=1e-10, rtol=1e-10) as it:
+        it.set_coef(pert)
+        outs, nxt = [], [0]
+
+        def cb(t, h, ok, cur):
+            while ok and nxt[0] < len(t_out) and t >= t_out[nxt[0]]:
+                outs.append(it.eval(t_out[nxt[0]]))
+                nxt[0] += 1
+            return t_out[nxt[0]] if nxt[0] < len(t_out) else np.inf
+
+        it.run(0.0, 0.01, 10.0, stop=t_out[0], callback=cb)
+        target = np.concatenate(outs)
+        it.set_coef(coef)
+        it.set_state(y0)
+        a0 = time.perf_counter()
+        hs = it.schedule(0.0, 0.01, 10.0)
+        a1 = time.perf_counter()
+        it.loss_grad(y0, 0.0, hs, t_out, target)  # warm (companion problem setup)
+        a2 = time.perf_counter()
+        L = it.loss_grad(y0, 0.0, hs, t_out, target, grad=False)
+        a3 = time.perf_counter()
+        L2, gc, gy = it.loss_grad(y0, 0.0, hs, t_out, target)
+        a4 = time.perf_counter()
+        errs = []
+        for t in (0, 3, 13):
+            cp, cm = coef.copy(), coef.copy()
+            cp[t] += 1e-6
+            cm[t] -= 1e-6
+            it.set_coef(cp)
+            Lp = it.loss_grad(y0, 0.0, hs, t_out, target, grad=False)
+            it.set_coef(cm)
+            Lm = it.loss_grad(y0, 0.0, hs, t_out, target, grad=False)
+            errs.append(abs((Lp - Lm) / 2e-6 - gc[t]) / max(abs(gc[t]), 1e-300))
+    out = {"workload": "2-D CGLE 256x256, L=100, t=10, 20 outputs, rtol=atol=1e-10, 5% "
+                       "coefficient noise target", "steps": int(hs.size),
+           "schedule_s": a1 - a0, "loss_s": a3 - a2, "loss_grad_s": a4 - a3,
+           "grad_over_loss": (a4 - a3) / (a3 - a2), "loss": L2,
+           "fd_max_rel_err_3coef": max(errs), "grad_norm_coef": float(np.linalg.norm(gc))}
+    log("adjoint:", json.dumps(out))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +241,8 @@ def main():
                     help="end time of every sweep instance")
     ap.add_argument("--serial-every", type=int, default=16,
                     help="serial baseline on every k-th sweep instance")
+    ap.add_argument("--adjoint", type=int, default=1,
+                    help="1: time the discrete adjoint on 2-D CGLE 256x256, t=10 (0 to skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each instance lives "
@@ -292,6 +354,8 @@ def main():
     it.close()
     if args.sweep > 0 and nd == 2:
         result["batched"] = sweep_leg(lib, args.sweep, args.sweep_t, args.serial_every)
+    if args.adjoint and nd == 2:
+        result["adjoint"] = adjoint_leg(lib)
     print(json.dumps(result), flush=True)
 
 
