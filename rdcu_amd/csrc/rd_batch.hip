@@ -95,7 +95,7 @@ void launch_col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, long gri
   cg.ntc = (c->Nxh + tc - 1) / tc;
   cg.tc = tc;
   cg.gs1 = c->Nh;
-  cg.es = c->Nxh;
+  cg.es = c->Nxp;
   cg.gdiv = 1;
   cg.nparts = (cl.max_fpf + fpw - 1) / fpw;
   const int per = c->n * cg.ntc * cg.nparts;

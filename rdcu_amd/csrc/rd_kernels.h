@@ -61,7 +61,8 @@ struct Grid {
   int n, ndim;
   int Ns[3];
   int Nx, Nxh, L;
-  long N, rows, Nh;
+  int Nxp;  // row pitch of half spectra (complex): Nxh rounded up to 8 (128 B)
+  long N, rows, Nh;  // Nh = rows * Nxp
   double invN;
   const double *kt[3];  // wavenumbers per axis (device)
 };
@@ -269,7 +270,7 @@ __global__ void k_row_r2c(Grid g, FftPlan px, const cx *__restrict__ twN, DevSta
     const cx Bv = cconj(Z[rr * L + (k == 0 ? 0 : L - k)]);
     const cx E = cadd(Av, Bv), D = csub(Av, Bv);
     const cx t = cmul(D, twN[k]);
-    b.S[(row0 + rr) * Nxh + k] = cx{0.5 * (E.re + t.im), 0.5 * (E.im - t.re)};
+    b.S[(row0 + rr) * g.Nxp + k] = cx{0.5 * (E.re + t.im), 0.5 * (E.im - t.re)};
   }
 }
 
@@ -412,7 +413,7 @@ __global__ void k_row_c2r(Grid g, FftPlan px, const cx *__restrict__ twN, DevSta
         Av = apply_mult(g, blk, &nk0, row[k]);
         Bv = apply_mult(g, blk, &nk1, row[L - k]);
       } else {
-        const cx *row = b.F + (long)m * g.Nh + r * Nxh;
+        const cx *row = b.F + (long)m * g.Nh + r * g.Nxp;
         Av = row[k];
         Bv = row[L - k];
       }

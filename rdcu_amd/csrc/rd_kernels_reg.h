@@ -210,7 +210,7 @@ __device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan
   for (int q = 0; q < E; ++q) lb[t + T * q] = v[q];
   __syncthreads();
   if (!act) return;
-  cx *S = b.S + row * Nxh;
+  cx *S = b.S + row * g.Nxp;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int k = t + T * q;
@@ -351,7 +351,7 @@ __device__ __forceinline__ void c2r_pre(const Grid &g, const FacList &fl, const 
                                         cx (&v)[E]) {
   const int L = g.L;
   const int blk = fl.fblk[m];
-  const cx *row = g.ndim == 1 ? b.S + (long)fl.ffield[m] * g.Nh : b.F + (long)m * g.Nh + r * g.Nxh;
+  const cx *row = g.ndim == 1 ? b.S + (long)fl.ffield[m] * g.Nh : b.F + (long)m * g.Nh + r * g.Nxp;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int k = t + T * q;
@@ -565,7 +565,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   for (int q = 0; q < E; ++q) lb[t + T * q] = v[q];
   __syncthreads();
   if (!act) return;
-  cx *S = b.S + ((long)f * g.rows + r) * Nxh;
+  cx *S = b.S + ((long)f * g.rows + r) * g.Nxp;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int k = t + T * q;

@@ -116,7 +116,7 @@ struct rdcu_ctx {
   int n = 0, ndim = 0, Ns[3] = {1, 1, 1};
   double Ls[3] = {1, 1, 1};
   long N = 0, rows = 0, Nh = 0;
-  int Nx = 0, Nxh = 0, L = 0, nblk = 0;
+  int Nx = 0, Nxh = 0, Nxp = 0, L = 0, nblk = 0;
   int nf_used = 0, nf_all = 0;
   rd::Grid grid{};
   rd::FftPlan px{}, p1{}, p0{};  // x, axis 1 (3D only), axis 0 (ndim >= 2)
@@ -239,7 +239,7 @@ std::vector<Launch> rhs_launches(rdcu_ctx *c, int stage, int src_kind, const dou
   const int nd = c->ndim;
   const int ntc = (c->Nxh + c->col_tc - 1) / c->col_tc;
   if (nd == 3) {
-    rd::ColGeom cg{c->Ns[1], (long)c->Nxh, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxh, ntc,
+    rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc,
                    c->col_tc, 1};
     dim3 gg((unsigned)((long)c->n * c->Ns[0] * ntc));
     v.push_back({"col_fwd", [=]() {
@@ -255,13 +255,13 @@ std::vector<Launch> rhs_launches(rdcu_ctx *c, int stage, int src_kind, const dou
     cg.axis = 0;
     cg.gs1 = c->Nh;
     if (nd == 2) {
-      cg.es = c->Nxh;
+      cg.es = c->Nxp;
       cg.gdiv = 1;
       cg.gs2 = 0;
     } else {
-      cg.es = (long)c->Ns[1] * c->Nxh;
+      cg.es = (long)c->Ns[1] * c->Nxp;
       cg.gdiv = c->Ns[1];
-      cg.gs2 = c->Nxh;
+      cg.gs2 = c->Nxp;
     }
     dim3 gg((unsigned)((long)c->n * cg.gdiv * ntc));
     v.push_back({"col_mult", [=]() {
@@ -270,7 +270,7 @@ std::vector<Launch> rhs_launches(rdcu_ctx *c, int stage, int src_kind, const dou
                  }});
   }
   if (nd == 3 && c->nf_used > 0) {
-    rd::ColGeom cg{c->Ns[1], (long)c->Nxh, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxh, ntc,
+    rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc,
                    c->col_tc, 1};
     dim3 gg((unsigned)((long)c->nf_used * c->Ns[0] * ntc));
     v.push_back({"col_inv", [=]() {
@@ -419,7 +419,7 @@ std::vector<Launch> col_launches_reg(rdcu_ctx *c, int check, int tc) {
   const int nd = c->ndim;
   const int ntc = (c->Nxh + tc - 1) / tc;
   if (nd == 3) {
-    rd::ColGeom cg{c->Ns[1], (long)c->Nxh, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxh, ntc, tc, 1};
+    rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc, tc, 1};
     long ng = (long)c->n * c->Ns[0];
     v.push_back({"col_fwd", [=]() { colr(c, c->p1, c->E1, cg, ng, check, c->fl_used, 0); }});
   }
@@ -430,18 +430,18 @@ std::vector<Launch> col_launches_reg(rdcu_ctx *c, int check, int tc) {
     cg.tc = tc;
     cg.gs1 = c->Nh;
     if (nd == 2) {
-      cg.es = c->Nxh;
+      cg.es = c->Nxp;
       cg.gdiv = 1;
     } else {
-      cg.es = (long)c->Ns[1] * c->Nxh;
+      cg.es = (long)c->Ns[1] * c->Nxp;
       cg.gdiv = c->Ns[1];
-      cg.gs2 = c->Nxh;
+      cg.gs2 = c->Nxp;
     }
     long ng = (long)c->n * cg.gdiv;
     v.push_back({"col_mult", [=]() { colr(c, c->p0, c->E0, cg, ng, check, c->fl_used, 1); }});
   }
   if (nd == 3 && c->nf_used > 0) {
-    rd::ColGeom cg{c->Ns[1], (long)c->Nxh, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxh, ntc, tc, 1};
+    rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc, tc, 1};
     long ng = (long)c->nf_used * c->Ns[0];
     v.push_back({"col_inv", [=]() { colr(c, c->p1, c->E1, cg, ng, check, c->fl_used, 2); }});
   }
@@ -786,19 +786,19 @@ int enqueue_all_factors(rdcu_ctx *c) {
     cg.tc = c->col_tc;
     cg.gs1 = c->Nh;
     if (nd == 2) {
-      cg.es = c->Nxh;
+      cg.es = c->Nxp;
       cg.gdiv = 1;
     } else {
-      cg.es = (long)c->Ns[1] * c->Nxh;
+      cg.es = (long)c->Ns[1] * c->Nxp;
       cg.gdiv = c->Ns[1];
-      cg.gs2 = c->Nxh;
+      cg.gs2 = c->Nxp;
     }
     dim3 gg((unsigned)((long)c->n * cg.gdiv * ntc));
     hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream, c->grid, c->p0, cg,
                        c->st, 0, c->bufs, c->fl_all, 1);
   }
   if (nd == 3) {
-    rd::ColGeom cg{c->Ns[1], (long)c->Nxh, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxh, ntc,
+    rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc,
                    c->col_tc, 1};
     dim3 gg((unsigned)((long)c->nf_all * c->Ns[0] * ntc));
     hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream, c->grid, c->p1, cg,
@@ -951,8 +951,9 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
   c->Nx = c->Ns[p->ndim - 1];
   c->L = c->Nx / 2;
   c->Nxh = c->L + 1;
+  c->Nxp = (c->Nxh + 7) / 8 * 8;
   c->rows = c->N / c->Nx;
-  c->Nh = c->rows * c->Nxh;
+  c->Nh = c->rows * c->Nxp;
   c->nblk = 1 + c->ndim + c->ndim * c->ndim;
   c->fixed = fixedstep ? 1 : 0;
 
@@ -1035,6 +1036,7 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
   for (int a = 0; a < 3; ++a) g.Ns[a] = c->Ns[a];
   g.Nx = c->Nx;
   g.Nxh = c->Nxh;
+  g.Nxp = c->Nxp;
   g.L = c->L;
   g.N = c->N;
   g.rows = c->rows;
