@@ -24,7 +24,7 @@ struct BatchClass {
   int off = 0;               // offset of this class in the batch's instance table
   rdcu_ctx *c0 = nullptr;    // a representative member (geometry, plans, tables)
   int Ex = 0, E0 = 0, rows = 0, nf_max = 0, nwords_max = 0, max_fpf = 1;
-  int rpb = 1, tc = 1, fpw = 1, G = 1;
+  int rpb = 1, tc = 1, fpw = 1, G = 1, R = 1;
   long grid_rf = 0, grid_col = 0;  // persistent grid sizes (0: one block per item)
   std::string tune_log;
 };
@@ -67,10 +67,13 @@ namespace {
 
 int ninst(const rdcu_batch *b) { return (int)b->members.size(); }
 
-size_t rowi_lds(const BatchClass &cl, int G) {
+// LDS row region of the class (largest factor count and program).
+long rowi_region_b(const BatchClass &cl, int G) {
   const rdcu_ctx *c = cl.c0;
-  return (size_t)(c->L + G * rd::padded_len(c->L)) * sizeof(cx) +
-         (size_t)(2 * c->n + cl.nf_max) * c->Nx * sizeof(double) + (size_t)cl.nwords_max * 4;
+  return rd::rowi_region(c->L, c->Nx, c->n, cl.nf_max, cl.nwords_max, G);
+}
+size_t rowi_lds(const BatchClass &cl, int G, int R) {
+  return (size_t)R * rowi_region_b(cl, G) * sizeof(cx);
 }
 
 template <int E>
@@ -112,27 +115,30 @@ void launch_col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, long gri
 
 template <int E>
 void launch_rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_kind, int dst_slot,
-                   int err, rd::StageCoef nx, int G, int check) {
+                   int err, rd::StageCoef nx, int G, int R, int check) {
   const rdcu_ctx *c = cl.c0;
   const int T = c->L / E;
   rd::ErrCoef ec{};
   for (int j = 0; j < 7; ++j) ec.e[j] = EE[j];
-  const size_t lds = rowi_lds(cl, G);
+  const size_t lds = rowi_lds(cl, G, R);
+  long region = rowi_region_b(cl, G);
   for (const BatchGroup &gr : cl.groups) {
     const rd::Inst *in = b->d_inst + cl.off + gr.i0;
     int cnt = gr.i1 - gr.i0, rows = (int)c->rows;
-    const unsigned grid = (unsigned)((long)cnt * rows);  // one row per block
+    const unsigned grid = (unsigned)((long)cnt * rows / R);  // R rows per block, R | rows
+    const unsigned block = (unsigned)(R * G * T);
     if (gr.fn) {
       rd::FftPlan px = c->px;
       const rd::cx *tw = c->twN;
       int a_src = src_kind, a_dk = dst_kind, a_ds = dst_slot, a_err = err, a_G = G, a_chk = check;
       void *args[] = {&in, &cnt, &rows, &px, &tw, &a_src, &a_dk, &a_ds, &a_err, &ec, &nx, &a_G,
-                      &a_chk};
-      (void)hipModuleLaunchKernel(gr.fn, grid, 1, 1, (unsigned)(G * T), 1, 1, (unsigned)lds,
-                                  b->stream, args, nullptr);
+                      &a_chk, &region};
+      (void)hipModuleLaunchKernel(gr.fn, grid, 1, 1, block, 1, 1, (unsigned)lds, b->stream, args,
+                                  nullptr);
     } else {
-      hipLaunchKernelGGL(rd::k_rowi_b<E>, dim3(grid), dim3(G * T), lds, b->stream, in, cnt, rows,
-                         c->px, c->twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check);
+      hipLaunchKernelGGL(rd::k_rowi_b<E>, dim3(grid), dim3(block), lds, b->stream, in, cnt, rows,
+                         c->px, c->twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check,
+                         region);
     }
   }
 }
@@ -148,9 +154,9 @@ void col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, long grid, int 
   else launch_col_b<24>(b, cl, tc, fpw, grid, check);
 }
 void rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_kind, int dst_slot, int err,
-            rd::StageCoef nx, int G, int check) {
-  if (cl.Ex == 8) launch_rowi_b<8>(b, cl, src_kind, dst_kind, dst_slot, err, nx, G, check);
-  else launch_rowi_b<24>(b, cl, src_kind, dst_kind, dst_slot, err, nx, G, check);
+            rd::StageCoef nx, int G, int R, int check) {
+  if (cl.Ex == 8) launch_rowi_b<8>(b, cl, src_kind, dst_kind, dst_slot, err, nx, G, R, check);
+  else launch_rowi_b<24>(b, cl, src_kind, dst_kind, dst_slot, err, nx, G, R, check);
 }
 
 // One plain RHS of every instance of a class (input y, output k slot `slot`
@@ -159,7 +165,7 @@ void rhs_b(rdcu_batch *b, const BatchClass &cl, int dst_kind, int slot, int chec
   rd::StageCoef none{};
   rowf_b(b, cl, rd::SRC_Y, none, cl.rpb, cl.grid_rf, check);
   col_b(b, cl, cl.tc, cl.fpw, cl.grid_col, check);
-  rowi_b(b, cl, rd::SRC_Y, dst_kind, slot, 0, none, cl.G, check);
+  rowi_b(b, cl, rd::SRC_Y, dst_kind, slot, 0, none, cl.G, cl.R, check);
 }
 
 // One DP45 attempt of every active instance (the single-instance sequence of
@@ -171,7 +177,7 @@ int enqueue_attempt_b(rdcu_batch *b) {
       col_b(b, cl, cl.tc, cl.fpw, cl.grid_col, 1);
       rd::StageCoef nx = s < 7 ? stage_coef(s + 1) : rd::StageCoef{};
       const int dk = s < 7 ? rd::DST_KS : rd::DST_K7;
-      rowi_b(b, cl, rd::SRC_YT, dk, s - 1, s == 7 && !b->fixed, nx, cl.G, 1);
+      rowi_b(b, cl, rd::SRC_YT, dk, s - 1, s == 7 && !b->fixed, nx, cl.G, cl.R, 1);
     }
   }
   hipLaunchKernelGGL(rd::k_control_b, dim3((unsigned)ninst(b)), dim3(256), 0, b->stream, b->d_inst,
@@ -241,13 +247,17 @@ int autotune_b(rdcu_batch *b, BatchClass &cl) {
   }
   best = 1e30;
   for (int G = c->n; G <= std::max(c->n, cl.nf_max); ++G) {
-    if (G * T > 256 || rowi_lds(cl, G) > 160 * 1024) break;
-    double tm = time_of([&]() { rowi_b(b, cl, rd::SRC_Y, rd::DST_KS, 1, 0, none, G, 0); });
-    snprintf(line, sizeof line, "row_inv G=%d %.1fus; ", G, tm * 1e3);
-    cl.tune_log += line;
-    if (tm < best) {
-      best = tm;
-      cl.G = G;
+    if (G * T > 256) break;
+    for (int R = 1; R * G * T <= 256 && c->rows % R == 0 && (R == 1 || cl.Ex <= 8); R *= 2) {
+      if (rowi_lds(cl, G, R) > 160 * 1024) break;
+      double tm = time_of([&]() { rowi_b(b, cl, rd::SRC_Y, rd::DST_KS, 1, 0, none, G, R, 0); });
+      snprintf(line, sizeof line, "row_inv G=%d R=%d %.1fus; ", G, R, tm * 1e3);
+      cl.tune_log += line;
+      if (tm < best) {
+        best = tm;
+        cl.G = G;
+        cl.R = R;
+      }
     }
   }
   (void)hipEventDestroy(e0);
@@ -272,9 +282,10 @@ void build_jit_b(rdcu_batch *b) {
       src += "extern \"C\" __global__ __launch_bounds__(256) void rdcu_rowi_b_" + tag +
              "(const rd::Inst *__restrict__ in, int ninst, int rows, rd::FftPlan px, const rd::cx "
              "*__restrict__ twN, int src_kind, int dst_kind, int dst_slot, int err, rd::ErrCoef "
-             "ec, rd::StageCoef nx, int G, int check) {\n";
+             "ec, rd::StageCoef nx, int G, int check, long region) {\n";
       src += "  rd::rowi_batch<" + std::to_string(cl.Ex) + ", rd::JitDict_" + tag +
-             ">(in, ninst, rows, px, twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check);\n}\n";
+             ">(in, ninst, rows, px, twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check, "
+             "region);\n}\n";
     }
   }
   std::vector<char> code;
@@ -500,7 +511,7 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
   }
   build_jit_b(b.get());
   for (BatchClass &cl : b->classes) {
-    if (rowi_lds(cl, cl.c0->n) > 160 * 1024)
+    if (rowi_lds(cl, cl.c0->n, 1) > 160 * 1024)
       return fail(RDCU_EUNSUPPORTED, "batched instance too large for one block per row");
     const char *shp = getenv("RDCU_SHAPES");
     if (!(shp && *shp)) {
@@ -513,13 +524,14 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
       m->cr_tc = cl.tc;
       m->cr_fpw = cl.fpw;
       m->ri_G = cl.G;
+      m->ri_R = cl.R;
     }
     const char *vb = getenv("RDCU_VERBOSE");
     if (vb && *vb && *vb != '0')
       fprintf(stderr, "rdcu batch class %dx%d (%zu instances, %zu structures, jit %s): %s -> "
-                      "%d,%d,%d,%d grids %ld,%ld\n",
+                      "%d,%d,%d,%d,%d grids %ld,%ld\n",
               cl.c0->Ns[0], cl.c0->Ns[1], cl.members.size(), cl.groups.size(),
-              b->jit_log.c_str(), cl.tune_log.c_str(), cl.rpb, cl.tc, cl.fpw, cl.G, cl.grid_rf,
+              b->jit_log.c_str(), cl.tune_log.c_str(), cl.rpb, cl.tc, cl.fpw, cl.G, cl.R, cl.grid_rf,
               cl.grid_col);
   }
   HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));

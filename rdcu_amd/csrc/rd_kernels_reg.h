@@ -372,17 +372,24 @@ __device__ __forceinline__ void c2r_pre(const Grid &g, const FacList &fl, const 
   }
 }
 
+// LDS of one row of the x-inverse pass (complex units): twiddles, G FFT
+// lines, (n+nf)*Nx values, n*Nx stage results, the dictionary program.
+__host__ __device__ inline long rowi_region(int L, int Nx, int n, int nf, int nwords, int G) {
+  return (long)L + (long)G * padded_len(L) + ((long)(2 * n + nf) * Nx + 1) / 2 + (nwords + 3) / 4;
+}
+
+// One grid row r, by the nt threads tid = 0..nt-1 of a row group whose LDS
+// region starts at smem.  Every row group of a block runs the same barrier
+// sequence.
 template <int E, class DE>
 __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
                                           const FacList &fl, const DictP &dict, int src_kind,
                                           const double *src_ptr, int dst_kind, int dst_slot,
                                           double *dst_ptr, int err, const ErrCoef &ec,
-                                          const StageCoef &nx, int G) {
-  extern __shared__ cx smem[];
+                                          const StageCoef &nx, int G, cx *smem, int tid, int nt) {
   const int n = g.n, Nx = g.Nx, L = g.L, Nxh = g.Nxh, T = L / E;
   const long N = g.N;
-  const int tid = threadIdx.x, nt = blockDim.x;
   const int gl = tid / T, t = tid - gl * T;
   const int nf = fl.nf;
   const int PL = padded_len(L);
@@ -584,6 +591,29 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   }
 }
 
+// Block = R rows x G*T threads (R = blockDim / (G*T)), row r = blockIdx*R +
+// sub.  With 24 values per thread the per-row thread range and LDS offset
+// cost registers the kernel does not have, so those kernels take R = 1.
+template <int E, class DE>
+__device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
+                                          const cx *__restrict__ twN, DevState *st, const Bufs &b,
+                                          const FacList &fl, const DictP &dict, int src_kind,
+                                          const double *src_ptr, int dst_kind, int dst_slot,
+                                          double *dst_ptr, int err, const ErrCoef &ec,
+                                          const StageCoef &nx, int G) {
+  extern __shared__ cx smem[];
+  if constexpr (E <= 8) {
+    const int gt = G * (g.L / E), sub = threadIdx.x / gt;
+    rowi_body<E, DE>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl, dict,
+                     src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
+                     smem + sub * rowi_region(g.L, g.Nx, g.n, fl.nf, dict.nwords, G),
+                     threadIdx.x - sub * gt, gt);
+  } else {
+    rowi_body<E, DE>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
+                     dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x);
+  }
+}
+
 template <int E>
 __global__ __launch_bounds__(256) void k_rowi_reg(Grid g, FftPlan px, const cx *__restrict__ twN,
                                                   DevState *st, int check_active, Bufs b,
@@ -592,8 +622,8 @@ __global__ __launch_bounds__(256) void k_rowi_reg(Grid g, FftPlan px, const cx *
                                                   double *dst_ptr, int err, ErrCoef ec,
                                                   StageCoef nx, int G) {
   if (check_active && !attempt_active(st)) return;
-  rowi_body<E, ProgDict>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
-                         dst_slot, dst_ptr, err, ec, nx, G);
+  rowi_rows<E, ProgDict>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot,
+                         dst_ptr, err, ec, nx, G);
 }
 
 // ---------------------------------------------------------------- batches
@@ -642,23 +672,40 @@ __device__ __forceinline__ void rowi_batch(const Inst *__restrict__ in, int nins
                                            const FftPlan &px, const cx *__restrict__ twN,
                                            int src_kind, int dst_kind, int dst_slot, int err,
                                            const ErrCoef &ec, const StageCoef &nx, int G,
-                                           int check) {
-  const long it = blockIdx.x;
-  if (it >= (long)ninst * rows) return;
-  const int i = (int)(it / rows);
-  DevState *st = in[i].st;
-  if (check && !attempt_active(st)) return;
-  rowi_body<E, DE>(it - (long)i * rows, in[i].g, px, twN, st, in[i].b, in[i].fl, in[i].dict,
-                   src_kind, nullptr, dst_kind, dst_slot, nullptr, err, ec, nx, G);
+                                           int check, long region) {
+  extern __shared__ cx smem[];
+  // row regions are sized for the largest factor count and program of the class
+  if constexpr (E <= 8) {
+    const int gt = G * (px.L / E), R = blockDim.x / gt, sub = threadIdx.x / gt;
+    const long it = (long)blockIdx.x * R + sub;  // R divides rows: one instance per block
+    if (it >= (long)ninst * rows) return;
+    const int i = (int)(it / rows);
+    DevState *st = in[i].st;
+    if (check && !attempt_active(st)) return;
+    const Inst &I = in[i];
+    rowi_body<E, DE>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind, nullptr,
+                     dst_kind, dst_slot, nullptr, err, ec, nx, G, smem + sub * region,
+                     threadIdx.x - sub * gt, gt);
+  } else {  // one row per block (see rowi_rows)
+    const long it = blockIdx.x;
+    if (it >= (long)ninst * rows) return;
+    const int i = (int)(it / rows);
+    DevState *st = in[i].st;
+    if (check && !attempt_active(st)) return;
+    const Inst &I = in[i];
+    rowi_body<E, DE>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind, nullptr,
+                     dst_kind, dst_slot, nullptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x);
+  }
 }
 
 template <int E>
 __global__ __launch_bounds__(256) void k_rowi_b(const Inst *__restrict__ in, int ninst, int rows,
                                                 FftPlan px, const cx *__restrict__ twN,
                                                 int src_kind, int dst_kind, int dst_slot, int err,
-                                                ErrCoef ec, StageCoef nx, int G, int check) {
+                                                ErrCoef ec, StageCoef nx, int G, int check,
+                                                long region) {
   rowi_batch<E, ProgDict>(in, ninst, rows, px, twN, src_kind, dst_kind, dst_slot, err, ec, nx,
-                          G, check);
+                          G, check, region);
 }
 
 }  // namespace rd

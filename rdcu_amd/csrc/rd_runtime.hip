@@ -148,7 +148,7 @@ struct rdcu_ctx {
   // register-resident path: elements per thread for the x axis and the
   // strided axes (0 = not eligible); shapes picked by the autotuner
   int reg = 0, Ex = 0, E0 = 0, E1 = 0;
-  int rf_rpb = 1, cr_tc = 1, ri_G = 1, cr_fpw = 1, max_fpf = 1;
+  int rf_rpb = 1, cr_tc = 1, ri_G = 1, ri_R = 1, cr_fpw = 1, max_fpf = 1;
   size_t lds_rf = 0, lds_cr0 = 0, lds_cr1 = 0, lds_ri = 0;
   std::string tune_log;
   // host-side flags and counters
@@ -349,13 +349,13 @@ void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups
 
 template <int E>
 void launch_rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, int dst_kind,
-                 int dst_slot, double *dst_ptr, int err, rd::StageCoef nx, int G) {
+                 int dst_slot, double *dst_ptr, int err, rd::StageCoef nx, int G, int R) {
   const int T = c->L / E;
   rd::ErrCoef ec{};
   for (int j = 0; j < 7; ++j) ec.e[j] = EE[j];
-  size_t lds = (size_t)(c->L + G * rd::padded_len(c->L)) * sizeof(cx) +
-               (size_t)(2 * c->n + c->nf_used) * c->Nx * sizeof(double) +
-               (size_t)c->dictp.nwords * sizeof(int);
+  const size_t lds =
+      (size_t)R * rd::rowi_region(c->L, c->Nx, c->n, c->nf_used, c->dictp.nwords, G) * sizeof(cx);
+  const unsigned grid = (unsigned)(c->rows / R), block = (unsigned)(R * G * T);
   if (c->jit_rowi) {
     rd::Grid a_g = c->grid;
     rd::FftPlan a_px = c->px;
@@ -375,13 +375,13 @@ void launch_rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, in
     int a_G = G;
     void *args[] = {&a_g, &a_px, &a_tw, &a_st, &a_check, &a_b, &a_fl, &a_d, &a_src,
                     &a_sp, &a_dk, &a_ds, &a_dp, &a_err, &a_ec, &a_nx, &a_G};
-    (void)hipModuleLaunchKernel(c->jit_rowi, (unsigned)c->rows, 1, 1, (unsigned)(G * T), 1, 1,
-                                (unsigned)lds, c->stream, args, nullptr);
+    (void)hipModuleLaunchKernel(c->jit_rowi, grid, 1, 1, block, 1, 1, (unsigned)lds, c->stream,
+                                args, nullptr);
     return;
   }
-  hipLaunchKernelGGL(rd::k_rowi_reg<E>, dim3((unsigned)c->rows), dim3(G * T), lds, c->stream,
-                     c->grid, c->px, c->twN, c->st, check, c->bufs, c->fl_used, c->dictp, src_kind,
-                     src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G);
+  hipLaunchKernelGGL(rd::k_rowi_reg<E>, dim3(grid), dim3(block), lds, c->stream, c->grid, c->px,
+                     c->twN, c->st, check, c->bufs, c->fl_used, c->dictp, src_kind, src_ptr,
+                     dst_kind, dst_slot, dst_ptr, err, ec, nx, G);
 }
 
 void rowf(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, rd::StageCoef sc, int rpb) {
@@ -394,9 +394,11 @@ void colr(rdcu_ctx *c, const rd::FftPlan &p, int E, rd::ColGeom cg, long ngroups
   else launch_colr<24>(c, p, cg, ngroups, check, fl, mode);
 }
 void rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, int dst_kind, int dst_slot,
-          double *dst_ptr, int err, rd::StageCoef nx, int G) {
-  if (c->Ex == 8) launch_rowi<8>(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, nx, G);
-  else launch_rowi<24>(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, nx, G);
+          double *dst_ptr, int err, rd::StageCoef nx, int G, int R) {
+  if (c->Ex == 8)
+    launch_rowi<8>(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, nx, G, R);
+  else
+    launch_rowi<24>(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, nx, G, R);
 }
 
 rd::StageCoef stage_coef(int stage) {
@@ -456,7 +458,7 @@ std::vector<Launch> rhs_launches_reg(rdcu_ctx *c, int src_kind, const double *sr
   v.push_back({"row_fwd", [=]() { rowf(c, check, src_kind, src_ptr, none, c->rf_rpb); }});
   for (auto &l : col_launches_reg(c, check, c->cr_tc)) v.push_back(l);
   v.push_back({"row_inv", [=]() {
-                 rowi(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, 0, none, c->ri_G);
+                 rowi(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, 0, none, c->ri_G, c->ri_R);
                }});
   return v;
 }
@@ -474,7 +476,7 @@ std::vector<Launch> attempt_launches_reg(rdcu_ctx *c) {
     int dk = s < 7 ? rd::DST_KS : rd::DST_K7;
     int err = s == 7 && !c->fixed;
     v.push_back({"row_inv", [=]() {
-                   rowi(c, 1, rd::SRC_YT, nullptr, dk, s - 1, nullptr, err, nx, c->ri_G);
+                   rowi(c, 1, rd::SRC_YT, nullptr, dk, s - 1, nullptr, err, nx, c->ri_G, c->ri_R);
                  }});
   }
   return v;
@@ -621,10 +623,11 @@ void build_jit(rdcu_ctx *c) {
          "const rd::cx *__restrict__ twN, rd::DevState *st, int check_active, rd::Bufs b, "
          "rd::FacList fl, rd::DictP dict, int src_kind, const double *src_ptr, int dst_kind, "
          "int dst_slot, double *dst_ptr, int err, rd::ErrCoef ec, rd::StageCoef nx, int G) {\n";
+  const std::string E = std::to_string(c->Ex);
   src += "  if (check_active && !rd::attempt_active(st)) return;\n";
-  src += "  rd::rowi_body<" + std::to_string(c->Ex) +
-         ", rd::JitDict>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, "
-         "dst_slot, dst_ptr, err, ec, nx, G);\n}\n";
+  src += "  rd::rowi_rows<" + E +
+         ", rd::JitDict>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot, "
+         "dst_ptr, err, ec, nx, G);\n}\n";
   std::vector<char> code;
   if (!jit_compile(src, jit_arch(c->dev), code, c->jit_log)) return;
   if (hipModuleLoadData(&c->jit_mod, code.data()) != hipSuccess ||
@@ -644,18 +647,21 @@ void build_jit(rdcu_ctx *c) {
 // current state.
 // Shapes given by the caller instead of the autotuner (profiling runs that
 // must see only the launches of one shape), checked against the same limits.
+size_t rowi_lds(const rdcu_ctx *c, int G, int R) {
+  return (size_t)R * rd::rowi_region(c->L, c->Nx, c->n, c->nf_used, c->dictp.nwords, G) *
+         sizeof(cx);
+}
+
 int fixed_shapes(rdcu_ctx *c, const char *spec) {
-  int rpb = 0, tc = 0, fpw = 0, G = 0;
-  if (sscanf(spec, "%d,%d,%d,%d", &rpb, &tc, &fpw, &G) != 4)
-    return fail(RDCU_EINVAL, "RDCU_SHAPES must be rpb,tc,fpw,G");
+  int rpb = 0, tc = 0, fpw = 0, G = 0, R = 1;
+  const int got = sscanf(spec, "%d,%d,%d,%d,%d", &rpb, &tc, &fpw, &G, &R);
+  if (got != 4 && got != 5) return fail(RDCU_EINVAL, "RDCU_SHAPES must be rpb,tc,fpw,G[,R]");
   const int T = c->L / c->Ex;
   int Lmax = c->Ns[0], Emin = c->E0;
   if (c->ndim == 3) {
     Lmax = std::max(Lmax, c->Ns[1]);
     Emin = std::min(c->E0, c->E1);
   }
-  const size_t ri_lds = (size_t)(c->L + G * rd::padded_len(c->L)) * sizeof(cx) +
-                        (size_t)(2 * c->n + c->nf_used) * c->Nx * 8 + (size_t)c->dictp.nwords * 4;
   const bool ok =
       rpb >= 1 && rpb * T <= 256 &&
       (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx) <= 160 * 1024 &&
@@ -663,12 +669,15 @@ int fixed_shapes(rdcu_ctx *c, const char *spec) {
                        (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) <=
                            160 * 1024 &&
                        fpw >= 1 && fpw <= std::max(1, c->max_fpf))) &&
-      G >= c->n && G <= std::max(c->n, c->nf_used) && G * T <= 256 && ri_lds <= 160 * 1024;
+      G >= c->n && G <= std::max(c->n, c->nf_used) && R >= 1 && (R == 1 || c->Ex <= 8) &&
+      R * G * T <= 256 &&
+      c->rows % R == 0 && rowi_lds(c, G, R) <= 160 * 1024;
   if (!ok) return fail(RDCU_EINVAL, std::string("RDCU_SHAPES out of range: ") + spec);
   c->rf_rpb = rpb;
   c->cr_tc = tc;
   c->cr_fpw = fpw;
   c->ri_G = G;
+  c->ri_R = R;
   c->tune_log = std::string("fixed ") + spec;
   return RDCU_OK;
 }
@@ -734,21 +743,23 @@ int autotune(rdcu_ctx *c) {
     }
     c->cr_fpw = best_fpw;
   }
-  // x inverse: factor lines per group
+  // x inverse: factor lines per group, rows per block (dividing the rows)
   best = 1e30;
   for (int G = c->n; G <= std::max(c->n, c->nf_used); ++G) {
     if (G * T > 256) break;
-    size_t lds = (size_t)(c->L + G * rd::padded_len(c->L)) * sizeof(cx) +
-                 (size_t)(2 * c->n + c->nf_used) * c->Nx * 8 + (size_t)c->dictp.nwords * 4;
-    if (lds > 160 * 1024) break;
-    double tm = time_of([&]() {
-      rowi(c, 0, rd::SRC_Y, nullptr, rd::DST_PTR, 0, c->rhs_out, 0, none, G);
-    });
-    snprintf(line, sizeof line, "row_inv G=%d %.2fus; ", G, tm * 1e3);
-    c->tune_log += line;
-    if (tm < best) {
-      best = tm;
-      c->ri_G = G;
+    for (int R = 1; R * G * T <= 256 && (R == 1 || c->Ex <= 8); R *= 2) {
+      if (c->rows % R) break;
+      if (rowi_lds(c, G, R) > 160 * 1024) break;
+      double tm = time_of([&]() {
+        rowi(c, 0, rd::SRC_Y, nullptr, rd::DST_PTR, 0, c->rhs_out, 0, none, G, R);
+      });
+      snprintf(line, sizeof line, "row_inv G=%d R=%d %.2fus; ", G, R, tm * 1e3);
+      c->tune_log += line;
+      if (tm < best) {
+        best = tm;
+        c->ri_G = G;
+        c->ri_R = R;
+      }
     }
   }
   (void)hipEventDestroy(e0);
@@ -1196,8 +1207,9 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
     }
     const char *vb = getenv("RDCU_VERBOSE");
     if (vb && *vb && *vb != '0')
-      fprintf(stderr, "rdcu jit: %s\nrdcu autotune: %s\nrdcu shapes: %d,%d,%d,%d\n",
-              c->jit_log.c_str(), c->tune_log.c_str(), c->rf_rpb, c->cr_tc, c->cr_fpw, c->ri_G);
+      fprintf(stderr, "rdcu jit: %s\nrdcu autotune: %s\nrdcu shapes: %d,%d,%d,%d,%d\n",
+              c->jit_log.c_str(), c->tune_log.c_str(), c->rf_rpb, c->cr_tc, c->cr_fpw, c->ri_G,
+              c->ri_R);
   }
 
   // capture one attempt
