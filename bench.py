@@ -119,6 +119,12 @@ def sweep_leg(lib, count, t_end, every):
     b.close()
     log(f"batched: {n} instances, setup {c1 - c0:.2f} s, run {c2 - c1:.3f} s, "
         f"attempts {int(na.sum())} (max {int(na.max())})")
+    for gsz in sorted({x.grid for x in insts}):
+        for kind in ("cgle", "fhn"):
+            a = [int(na[i]) for i, x in enumerate(insts) if x.grid == gsz and x.kind == kind]
+            if a:
+                log(f"  {kind} {gsz}^2: {len(a)} instances, attempts mean {np.mean(a):.0f} "
+                    f"max {max(a)}")
     # warm the plain path's per-process caches (kernel compilation per
     # dictionary structure and transform length) before timing
     warm = {}

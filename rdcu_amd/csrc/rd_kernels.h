@@ -55,6 +55,7 @@ struct Bufs {
   cx *F;
   double *part;   // per-line sums of squares
   double *part2;  // block sums when there are more than kRedCap lines
+  unsigned *cnt;  // blocks of the error pass done (step control fused into the last)
 };
 
 struct Grid {
@@ -375,6 +376,8 @@ __global__ void k_col(Grid g, FftPlan p, ColGeom cg, DevState *st, int check_act
 // Dynamic LDS: vals (n+nf)*Nx doubles, then 2*grp*L complex of FFT scratch.
 struct ErrCoef {
   double e[7];
+  int fuse_ctl;    // the last block of the error pass runs the step control
+  long nlines, nN;
 };
 
 __global__ void k_row_c2r(Grid g, FftPlan px, const cx *__restrict__ twN, DevState *st,
@@ -539,9 +542,9 @@ __global__ void k_reduce(const double *__restrict__ in, long cnt, double *out, D
 }
 
 // Error norm and step control (dp45.cu:117-146) on the device.
-__device__ __forceinline__ void control_body(DevState *st, const double *__restrict__ part,
-                                             long cnt, long nN) {
-  __shared__ double v[kRedCap];
+// v: LDS for cnt doubles.  Every thread of the block calls it.
+__device__ __forceinline__ void control_step(DevState *st, const double *__restrict__ part,
+                                             long cnt, long nN, double *v) {
   if (!attempt_active(st)) return;
   const int fixed = st->fixed;
   if (!fixed) {
@@ -606,6 +609,12 @@ __device__ __forceinline__ void control_body(DevState *st, const double *__restr
   st->log_acc[i] = acc ? 1 : 0;
   st->nlog = i + 1;
   if (i + 1 >= kLogCap) st->halted = 1;
+}
+
+__device__ __forceinline__ void control_body(DevState *st, const double *__restrict__ part,
+                                             long cnt, long nN) {
+  __shared__ double v[kRedCap];
+  control_step(st, part, cnt, nN, v);
 }
 
 __global__ void k_control(DevState *st, const double *__restrict__ part, long cnt, long nN) {

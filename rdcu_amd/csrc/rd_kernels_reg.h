@@ -523,21 +523,22 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   }
   if (nx.ns == 0) return;
 
-  // fused next stage for this row: ytemp = y + h*sum a_j k_j (k_s from LDS)
+  // fused next stage for this row: ytemp = y + h*sum a_j k_j (k_s from LDS),
+  // spread over all threads of the row (two values each step), staged in LDS
+  // (the value rows are free now) for the x forward pass of the n fields.
   __syncthreads();
-  const bool act = gl < n;
-  const int f = act ? gl : 0;
-  cx v[E];
-  if (act) {
+  double *ys = vals;
+  {
     const double hs = st->h;
     const double *y = b.y[st->iy];
     double *yt = b.y[st->iyt];
     const double *ks[6];
     ks[0] = b.k[st->k1s];
     for (int j = 1; j < 6; ++j) ks[j] = b.k[j];
-#pragma unroll
-    for (int q = 0; q < E; ++q) {
-      const int j2 = 2 * (t + T * q);
+    const int half = Nx / 2;
+#pragma unroll 4
+    for (int w = tid; w < n * half; w += nt) {
+      const int f = w / half, j2 = 2 * (w - f * half);
       const long off = f * N + r * Nx + j2;
       const double2 yv2 = *reinterpret_cast<const double2 *>(y + off);
       const double2 a1 = *reinterpret_cast<const double2 *>(ks[0] + off);
@@ -561,11 +562,17 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
       o.x = __builtin_fma(hs, ar, yv2.x);
       o.y = __builtin_fma(hs, ai, yv2.y);
       *reinterpret_cast<double2 *>(yt + off) = o;
-      v[q] = cx{o.x, o.y};
+      *reinterpret_cast<double2 *>(ys + f * Nx + j2) = o;
     }
-  } else {
+  }
+  __syncthreads();
+  const bool act = gl < n;
+  const int f = act ? gl : 0;
+  cx v[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = cx{0.0, 0.0};
+  for (int q = 0; q < E; ++q) {
+    const double2 o = *reinterpret_cast<const double2 *>(ys + f * Nx + 2 * (t + T * q));
+    v[q] = act ? cx{o.x, o.y} : cx{0.0, 0.0};
   }
   reg_fft<E, -1>(v, lb, 1, tw, px, t, T);
 #pragma unroll
@@ -611,6 +618,25 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
   } else {
     rowi_body<E, DE>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
                      dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x);
+  }
+  // Error pass of an attempt: the last block to finish reduces the line sums
+  // (the same pairwise tree) and runs the step control, in place of a
+  // separate launch.  The agent-scope fences publish and acquire the line
+  // sums of every block across the XCDs.  (8-value kernels only: the
+  // 24-value ones have no registers to spare for it.)
+  if (E <= 8 && err && ec.fuse_ctl) {
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      last = atomicAdd(b.cnt, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+      __threadfence();
+      if (threadIdx.x == 0) *b.cnt = 0;
+      control_step(st, b.part, ec.nlines, ec.nN, reinterpret_cast<double *>(smem));
+    }
   }
 }
 
