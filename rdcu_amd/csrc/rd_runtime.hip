@@ -348,6 +348,12 @@ void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups
                      check, c->bufs, fl, mode, fpw);
 }
 
+// R row regions plus one word for the fused step control's flag.
+size_t rowi_lds(const rdcu_ctx *c, int G, int R) {
+  return ((size_t)R * rd::rowi_region(c->L, c->Nx, c->n, c->nf_used, c->dictp.nwords, G) + 1) *
+         sizeof(cx);
+}
+
 template <int E>
 void launch_rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, int dst_kind,
                  int dst_slot, double *dst_ptr, int err, rd::StageCoef nx, int G, int R) {
@@ -357,8 +363,7 @@ void launch_rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, in
   ec.fuse_ctl = err && c->ctl_fused;
   ec.nlines = c->nlines;
   ec.nN = (long)c->n * c->N;
-  const size_t lds =
-      (size_t)R * rd::rowi_region(c->L, c->Nx, c->n, c->nf_used, c->dictp.nwords, G) * sizeof(cx);
+  const size_t lds = rowi_lds(c, G, R);
   const unsigned grid = (unsigned)(c->rows / R), block = (unsigned)(R * G * T);
   if (c->jit_rowi) {
     rd::Grid a_g = c->grid;
@@ -651,10 +656,6 @@ void build_jit(rdcu_ctx *c) {
 // current state.
 // Shapes given by the caller instead of the autotuner (profiling runs that
 // must see only the launches of one shape), checked against the same limits.
-size_t rowi_lds(const rdcu_ctx *c, int G, int R) {
-  return (size_t)R * rd::rowi_region(c->L, c->Nx, c->n, c->nf_used, c->dictp.nwords, G) *
-         sizeof(cx);
-}
 
 int fixed_shapes(rdcu_ctx *c, const char *spec) {
   int rpb = 0, tc = 0, fpw = 0, G = 0, R = 1;
@@ -1220,7 +1221,8 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
   }
 
   c->ctl_fused = c->reg && c->Ex == 8 && !c->fixed && c->nlines <= rd::kRedCap &&
-                 rowi_lds(c.get(), c->ri_G, c->ri_R) >= (size_t)c->nlines * sizeof(double);
+                 rowi_lds(c.get(), c->ri_G, c->ri_R) - sizeof(cx) >=
+                     (size_t)c->nlines * sizeof(double);  // line sums clear of the flag word
   // capture one attempt
   HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   st = enqueue_attempt(c.get());
