@@ -55,7 +55,6 @@ struct Bufs {
   cx *F;
   double *part;   // per-line sums of squares
   double *part2;  // block sums when there are more than kRedCap lines
-  unsigned *cnt;  // blocks of the error pass done (step control fused into the last)
 };
 
 struct Grid {
@@ -376,8 +375,6 @@ __global__ void k_col(Grid g, FftPlan p, ColGeom cg, DevState *st, int check_act
 // Dynamic LDS: vals (n+nf)*Nx doubles, then 2*grp*L complex of FFT scratch.
 struct ErrCoef {
   double e[7];
-  int fuse_ctl;    // the last block of the error pass runs the step control
-  long nlines, nN;
 };
 
 __global__ void k_row_c2r(Grid g, FftPlan px, const cx *__restrict__ twN, DevState *st,

@@ -609,8 +609,8 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
                                           double *dst_ptr, int err, const ErrCoef &ec,
                                           const StageCoef &nx, int G) {
   extern __shared__ cx smem[];
-  const long region = rowi_region(g.L, g.Nx, g.n, fl.nf, dict.nwords, G);
   if constexpr (E <= 8) {
+    const long region = rowi_region(g.L, g.Nx, g.n, fl.nf, dict.nwords, G);
     const int gt = G * (g.L / E), sub = threadIdx.x / gt;
     rowi_body<E, DE>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl, dict,
                      src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
@@ -618,26 +618,6 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
   } else {
     rowi_body<E, DE>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
                      dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x);
-  }
-  // Error pass of an attempt: the last block to finish reduces the line sums
-  // (the same pairwise tree) and runs the step control, in place of a
-  // separate launch.  The agent-scope fences publish and acquire the line
-  // sums of every block across the XCDs.  (8-value kernels only: the
-  // 24-value ones have no registers to spare for it.)
-  if (E <= 8 && err && ec.fuse_ctl) {
-    // one LDS word after the row regions (the launch adds it)
-    int *last = reinterpret_cast<int *>(smem + (blockDim.x / (G * (g.L / E))) * region);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __threadfence();
-      *last = atomicAdd(b.cnt, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (*last) {
-      __threadfence();
-      if (threadIdx.x == 0) *b.cnt = 0;
-      control_step(st, b.part, ec.nlines, ec.nN, reinterpret_cast<double *>(smem));
-    }
   }
 }
 

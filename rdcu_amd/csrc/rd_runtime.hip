@@ -149,7 +149,6 @@ struct rdcu_ctx {
   // strided axes (0 = not eligible); shapes picked by the autotuner
   int reg = 0, Ex = 0, E0 = 0, E1 = 0;
   int rf_rpb = 1, cr_tc = 1, ri_G = 1, ri_R = 1, cr_fpw = 1, max_fpf = 1;
-  bool ctl_fused = false;  // step control in the last block of the error pass
   size_t lds_rf = 0, lds_cr0 = 0, lds_cr1 = 0, lds_ri = 0;
   std::string tune_log;
   // host-side flags and counters
@@ -348,9 +347,8 @@ void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups
                      check, c->bufs, fl, mode, fpw);
 }
 
-// R row regions plus one word for the fused step control's flag.
 size_t rowi_lds(const rdcu_ctx *c, int G, int R) {
-  return ((size_t)R * rd::rowi_region(c->L, c->Nx, c->n, c->nf_used, c->dictp.nwords, G) + 1) *
+  return (size_t)R * rd::rowi_region(c->L, c->Nx, c->n, c->nf_used, c->dictp.nwords, G) *
          sizeof(cx);
 }
 
@@ -360,9 +358,6 @@ void launch_rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, in
   const int T = c->L / E;
   rd::ErrCoef ec{};
   for (int j = 0; j < 7; ++j) ec.e[j] = EE[j];
-  ec.fuse_ctl = err && c->ctl_fused;
-  ec.nlines = c->nlines;
-  ec.nN = (long)c->n * c->N;
   const size_t lds = rowi_lds(c, G, R);
   const unsigned grid = (unsigned)(c->rows / R), block = (unsigned)(R * G * T);
   if (c->jit_rowi) {
@@ -840,7 +835,6 @@ int enqueue_attempt(rdcu_ctx *c) {
     int st = enqueue_rhs(c, s, rd::SRC_STAGE, nullptr, dk, s - 1, nullptr, 1, s == 7 && !c->fixed);
     if (st) return st;
   }
-  if (c->reg && c->ctl_fused) return RDCU_OK;  // step control ran in the error pass
   const double *part = c->bufs.part;
   long cnt = c->nlines;
   if (!c->fixed && c->nlines > rd::kRedCap) {
@@ -1157,8 +1151,6 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
   c->nred = (c->nlines + rd::kRedCap - 1) / rd::kRedCap;
   if ((st = c->dalloc(&c->bufs.part, (size_t)c->nlines))) return st;
   if ((st = c->dalloc(&c->bufs.part2, (size_t)c->nred))) return st;
-  if ((st = c->dalloc(&c->bufs.cnt, 1))) return st;
-  HIPCHECK(hipMemset(c->bufs.cnt, 0, sizeof(unsigned)));
   if ((st = c->dalloc(&c->rhs_in, nN)) || (st = c->dalloc(&c->rhs_out, nN)) ||
       (st = c->dalloc(&c->yeval, nN)))
     return st;
@@ -1220,9 +1212,6 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
               c->ri_R);
   }
 
-  c->ctl_fused = c->reg && c->Ex == 8 && !c->fixed && c->nlines <= rd::kRedCap &&
-                 rowi_lds(c.get(), c->ri_G, c->ri_R) - sizeof(cx) >=
-                     (size_t)c->nlines * sizeof(double);  // line sums clear of the flag word
   // capture one attempt
   HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   st = enqueue_attempt(c.get());
