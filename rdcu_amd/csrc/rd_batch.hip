@@ -14,14 +14,16 @@
 // bit-identical to a run of that problem alone.
 
 struct BatchGroup {  // instances of one class sharing a dictionary structure
-  int i0 = 0, i1 = 0;  // range in the class's table
+  std::vector<int> all;  // its members
+  int i0 = 0, i1 = 0;    // range in the class's current table
   hipFunction_t fn = nullptr;  // specialised x-inverse kernel (null: interpreter)
 };
 
 struct BatchClass {
-  std::vector<int> members;  // batch member indices, in table order
+  std::vector<int> members;  // batch member indices (creation order)
   std::vector<BatchGroup> groups;
-  int off = 0;               // offset of this class in the batch's instance table
+  int off = 0;               // offset of this class in the batch's current instance table
+  int cnt = 0;               // instances of this class in the current table
   rdcu_ctx *c0 = nullptr;    // a representative member (geometry, plans, tables)
   int Ex = 0, E0 = 0, rows = 0, nf_max = 0, nwords_max = 0, max_fpf = 1;
   int rpb = 1, tc = 1, fpw = 1, G = 1, R = 1;
@@ -33,7 +35,9 @@ struct rdcu_batch {
   int dev = 0;
   hipStream_t stream = nullptr;
   std::vector<std::unique_ptr<rdcu_ctx>> members;
-  std::vector<int> pos;  // member -> position in the instance table
+  std::vector<int> pos;  // member -> device state slot (= position in the full table)
+  std::vector<int> tab;  // current table position -> member
+  bool table_full = true;
   std::vector<BatchClass> classes;
   rd::Inst *d_inst = nullptr;
   rd::DevState *d_st = nullptr, *h_st = nullptr;  // instance table order
@@ -82,11 +86,12 @@ void launch_rowf_b(rdcu_batch *b, const BatchClass &cl, int src_kind, rd::StageC
   const rdcu_ctx *c = cl.c0;
   const int T = c->L / E;
   const int per = (int)(((long)c->n * c->rows + rpb - 1) / rpb);
-  const long total = (long)cl.members.size() * per;
+  const long total = (long)cl.cnt * per;
+  if (total == 0) return;
   const long g = grid > 0 ? std::min(grid, total) : total;
   const size_t lds = (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx);
   hipLaunchKernelGGL(rd::k_rowf_b<E>, dim3((unsigned)g), dim3(rpb * T), lds, b->stream,
-                     b->d_inst + cl.off, (int)cl.members.size(), per, c->px, c->twN, src_kind,
+                     b->d_inst + cl.off, cl.cnt, per, c->px, c->twN, src_kind,
                      sc, rpb, check);
 }
 
@@ -105,12 +110,13 @@ void launch_col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, long gri
   cg.nblk = per;
   const int T = cg.L / E;
   (void)grid;
-  const long total = (long)cl.members.size() * per;
+  const long total = (long)cl.cnt * per;
+  if (total == 0) return;
   const long g = (total + 7) / 8 * 8;  // one item per block, a multiple of the 8 XCDs
   const size_t lds = (size_t)(cg.L + tc * rd::padded_len(cg.L) + tc * cg.L) * sizeof(cx) +
                      (size_t)cg.L * sizeof(double);
   hipLaunchKernelGGL(rd::k_col_b<E>, dim3((unsigned)g), dim3(tc * T), lds, b->stream,
-                     b->d_inst + cl.off, (int)cl.members.size(), per, c->p0, cg, 1, fpw, check);
+                     b->d_inst + cl.off, cl.cnt, per, c->p0, cg, 1, fpw, check);
 }
 
 template <int E>
@@ -125,6 +131,7 @@ void launch_rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_ki
   for (const BatchGroup &gr : cl.groups) {
     const rd::Inst *in = b->d_inst + cl.off + gr.i0;
     int cnt = gr.i1 - gr.i0, rows = (int)c->rows;
+    if (cnt == 0) continue;
     const unsigned grid = (unsigned)((long)cnt * rows / R);  // R rows per block, R | rows
     const unsigned block = (unsigned)(R * G * T);
     if (gr.fn) {
@@ -180,8 +187,8 @@ int enqueue_attempt_b(rdcu_batch *b) {
       rowi_b(b, cl, rd::SRC_YT, dk, s - 1, s == 7 && !b->fixed, nx, cl.G, cl.R, 1);
     }
   }
-  hipLaunchKernelGGL(rd::k_control_b, dim3((unsigned)ninst(b)), dim3(256), 0, b->stream, b->d_inst,
-                     b->d_done);
+  hipLaunchKernelGGL(rd::k_control_b, dim3((unsigned)b->tab.size()), dim3(256), 0, b->stream,
+                     b->d_inst, b->d_done);
   HIPCHECK(hipGetLastError());
   return RDCU_OK;
 }
@@ -275,7 +282,7 @@ void build_jit_b(rdcu_batch *b) {
   for (size_t ci = 0; ci < b->classes.size(); ++ci) {
     BatchClass &cl = b->classes[ci];
     for (size_t gi = 0; gi < cl.groups.size(); ++gi) {
-      const rdcu_ctx *m = b->members[cl.members[cl.groups[gi].i0]].get();
+      const rdcu_ctx *m = b->members[cl.groups[gi].all[0]].get();
       const std::string tag = std::to_string(ci) + "_" + std::to_string(gi);
       src += jit_dict_source(m->n, m->n + m->nf_used, m->h_tfield, m->h_tstart, m->h_fslot,
                              m->h_fpow, "JitDict_" + tag);
@@ -322,6 +329,61 @@ std::string structure_key(const rdcu_ctx *m) {
   add(m->h_fslot);
   add(m->h_fpow);
   return k;
+}
+
+// Instance table of the members with keep[i] set, class by class and group
+// by group (creation order within a group), uploaded to the device.
+int build_table(rdcu_batch *b, const std::vector<char> &keep) {
+  std::vector<rd::Inst> tab;
+  b->tab.clear();
+  bool full = true;
+  for (BatchClass &cl : b->classes) {
+    cl.off = (int)tab.size();
+    for (BatchGroup &gr : cl.groups) {
+      gr.i0 = (int)tab.size() - cl.off;
+      for (int i : gr.all) {
+        if (!keep[i]) {
+          full = false;
+          continue;
+        }
+        rdcu_ctx *m = b->members[i].get();
+        tab.push_back(rd::Inst{m->grid, m->bufs, m->st, m->fl_used, m->dictp});
+        b->tab.push_back(i);
+      }
+      gr.i1 = (int)tab.size() - cl.off;
+    }
+    cl.cnt = (int)tab.size() - cl.off;
+  }
+  b->table_full = full;
+  if (!tab.empty())
+    HIPCHECK(hipMemcpyAsync(b->d_inst, tab.data(), sizeof(rd::Inst) * tab.size(),
+                            hipMemcpyHostToDevice, b->stream));
+  HIPCHECK(hipStreamSynchronize(b->stream));
+  return RDCU_OK;
+}
+
+// Captures one attempt of the instances in the current table.
+int capture_b(rdcu_batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int st = enqueue_attempt_b(b);
+  hipGraph_t graph = nullptr;
+  hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+  if (st) return st;
+  if (ce != hipSuccess) return fail(RDCU_EDEVICE, std::string("capture: ") + hipGetErrorString(ce));
+  b->graph = graph;
+  HIPCHECK(hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0));
+  return RDCU_OK;
+}
+
+int full_table(rdcu_batch *b) {
+  if (b->table_full && b->gexec) return RDCU_OK;
+  int st = build_table(b, std::vector<char>(b->members.size(), 1));
+  if (st) return st;
+  return capture_b(b);
 }
 
 int read_states_b(rdcu_batch *b) {
@@ -449,8 +511,10 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
       gr.i0 = (int)cl.members.size();
       for (int i : sv.second) cl.members.push_back(i);
       gr.i1 = (int)cl.members.size();
+      gr.all = sv.second;
       cl.groups.push_back(gr);
     }
+    cl.cnt = (int)cl.members.size();
     for (int i : cl.members) final_order.push_back(i);
     off += (int)cl.members.size();
     b->classes.push_back(std::move(cl));
@@ -471,17 +535,12 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
     }
     HIPCHECK(hipMemcpy(b->d_st, b->h_st, sizeof(rd::DevState) * count, hipMemcpyHostToDevice));
   }
-  // instance table
+  // instance table (the full table's order is final_order: position = state slot)
   {
-    std::vector<rd::Inst> tab(count);
-    for (int p = 0; p < count; ++p) {
-      rdcu_ctx *m = b->members[final_order[p]].get();
-      tab[p] = rd::Inst{m->grid, m->bufs, m->st, m->fl_used, m->dictp};
-    }
     void *q = nullptr;
     HIPCHECK(hipMalloc(&q, sizeof(rd::Inst) * count));
     b->d_inst = static_cast<rd::Inst *>(q);
-    HIPCHECK(hipMemcpy(b->d_inst, tab.data(), sizeof(rd::Inst) * count, hipMemcpyHostToDevice));
+    if ((st = build_table(b.get(), std::vector<char>(count, 1)))) return st;
   }
   {
     static bool lds_raised = false;
@@ -534,14 +593,7 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
               b->jit_log.c_str(), cl.tune_log.c_str(), cl.rpb, cl.tc, cl.fpw, cl.G, cl.R, cl.grid_rf,
               cl.grid_col);
   }
-  HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  st = enqueue_attempt_b(b.get());
-  hipGraph_t graph = nullptr;
-  hipError_t ce = hipStreamEndCapture(b->stream, &graph);
-  if (st) return st;
-  if (ce != hipSuccess) return fail(RDCU_EDEVICE, std::string("capture: ") + hipGetErrorString(ce));
-  b->graph = graph;
-  HIPCHECK(hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0));
+  if ((st = capture_b(b.get()))) return st;
   HIPCHECK(hipStreamSynchronize(b->stream));
   *out = b.release();
   return RDCU_OK;
@@ -565,6 +617,7 @@ int rdcu_batch_step(rdcu_batch *b, double *t, double *h, double hmin, double hma
   if (!b || !t || !h || !accepted) return fail(RDCU_EINVAL, "null argument");
   HIPCHECK(hipSetDevice(b->dev));
   int st;
+  if ((st = full_table(b))) return st;
   if ((st = begin_b(b, t, h, nullptr, -INFINITY, hmax, 0))) return st;
   if ((st = ensure_k1_b(b, false))) return st;
   HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
@@ -589,6 +642,7 @@ int rdcu_batch_run(rdcu_batch *b, double *t, double *h, double hmin, double hmax
   std::vector<long long> a0(n);
   for (int i = 0; i < n; ++i) a0[i] = b->h_st[b->pos[i]].attempts;
   int st;
+  if ((st = full_table(b))) return st;
   if ((st = begin_b(b, t, h, t1, hmin, hmax, 1))) return st;
   // dp45_run computes k1 = f(y) at the start of a run (dp45.cu:157)
   if ((st = ensure_k1_b(b, true))) return st;
@@ -598,12 +652,24 @@ int rdcu_batch_run(rdcu_batch *b, double *t, double *h, double hmin, double hmax
     const long long k = std::min<long long>(kChunkMax, max_attempts - done);
     for (long long r = 0; r < k; ++r) HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
     done += k;
-    HIPCHECK(hipMemcpyAsync(b->h_done, b->d_done, sizeof(int) * n, hipMemcpyDeviceToHost,
+    const int tn = (int)b->tab.size();
+    HIPCHECK(hipMemcpyAsync(b->h_done, b->d_done, sizeof(int) * tn, hipMemcpyDeviceToHost,
                             b->stream));
     HIPCHECK(hipStreamSynchronize(b->stream));
     b->nsync++;
-    all = true;
-    for (int p = 0; p < n && all; ++p) all = b->h_done[p] != 0;
+    int active = 0;
+    std::vector<char> keep(n, 0);
+    for (int p = 0; p < tn; ++p)
+      if (!b->h_done[p]) {
+        ++active;
+        keep[b->tab[p]] = 1;
+      }
+    all = active == 0;
+    // once half the table has finished, later attempts launch only the
+    // instances still integrating (a new table and graph)
+    if (!all && done < max_attempts && 2 * active <= tn) {
+      if ((st = build_table(b, keep)) || (st = capture_b(b))) return st;
+    }
   }
   if ((st = read_states_b(b))) return st;
   for (int i = 0; i < n; ++i) {
