@@ -274,6 +274,25 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   cx v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = act ? src[(long)(t + T * q) * cg.es] : cx{0.0, 0.0};
+  // everything else the block reads from global memory is issued now, in the
+  // same round trip as the column data
+  const int nd = g.ndim;
+  const int kx = c0 + (act ? c : 0);
+  double kv[3];
+  int nk[3];
+  int blk0 = 0;
+  if (mode == 1) {
+    if (nd == 2) {
+      nk[1] = kx;
+      kv[1] = g.kt[1][kx];
+    } else {
+      nk[1] = g2;
+      kv[1] = g.kt[1][g2];
+      nk[2] = kx;
+      kv[2] = g.kt[2][kx];
+    }
+    blk0 = fl.fblk[m0];
+  }
   Prefetch<4, cx> pf_tw;
   pf_tw.load(p.tw, L, tid, blockDim.x);
   Prefetch<4, double> pf_k;
@@ -296,21 +315,8 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   cx *xs = buf + tc * padded_len(L);
 #pragma unroll
   for (int q = 0; q < E; ++q) xs[(t + T * q) * tc + c] = v[q];
-  const int nd = g.ndim;
-  double kv[3];
-  int nk[3];
-  const int kx = c0 + (act ? c : 0);
-  if (nd == 2) {
-    nk[1] = kx;
-    kv[1] = g.kt[1][kx];
-  } else {
-    nk[1] = g2;
-    kv[1] = g.kt[1][g2];
-    nk[2] = kx;
-    kv[2] = g.kt[2][kx];
-  }
   for (int m = m0; m < m1; ++m) {
-    const int blk = fl.fblk[m];
+    const int blk = m == m0 ? blk0 : fl.fblk[m];
     cx w[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -373,9 +379,9 @@ __device__ __forceinline__ void c2r_pre(const Grid &g, const FacList &fl, const 
 }
 
 // LDS of one row of the x-inverse pass (complex units): twiddles, G FFT
-// lines, (n+nf)*Nx values, n*Nx stage results, the dictionary program.
+// lines, (n+nf)*Nx values, the dictionary program.
 __host__ __device__ inline long rowi_region(int L, int Nx, int n, int nf, int nwords, int G) {
-  return (long)L + (long)G * padded_len(L) + ((long)(2 * n + nf) * Nx + 1) / 2 + (nwords + 3) / 4;
+  return (long)L + (long)G * padded_len(L) + ((long)(n + nf) * Nx + 1) / 2 + (nwords + 3) / 4;
 }
 
 // One grid row r, by the nt threads tid = 0..nt-1 of a row group whose LDS
@@ -395,8 +401,11 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   const int PL = padded_len(L);
   cx *tw = smem, *buf = smem + L;
   double *vals = reinterpret_cast<double *>(buf + G * PL);
-  double *fv = vals + (long)(n + nf) * Nx;
-  int *prog = reinterpret_cast<int *>(fv + (long)n * Nx);
+  // the stage results for the fused next stage share the FFT lines' LDS
+  // (free between the factor transforms and the next stage's transform; the
+  // error pass, which uses it for the squares, has no next stage)
+  double *fv = reinterpret_cast<double *>(buf);
+  int *prog = reinterpret_cast<int *>(vals + (long)(n + nf) * Nx);
   cx *lb = buf + gl * PL;
   const double *src =
       src_kind == SRC_Y ? b.y[st->iy] : (src_kind == SRC_YT ? b.y[st->iyt] : src_ptr);
@@ -492,7 +501,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
         if (x >= Nx) continue;
         const long idx = f * N + r * Nx + x;
         out[idx] = acc[i];
-        fv[f * Nx + x] = acc[i];
+        if (nx.ns) fv[f * Nx + x] = acc[i];
         if (err) {
           double e = ec.e[0] * ev[i][0];
           e = __builtin_fma(ec.e[2], ev[i][1], e);
