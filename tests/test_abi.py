@@ -52,3 +52,31 @@ def test_hip_cli_help_runs_without_gpu(tmp_path):
     r = subprocess.run([exe, "-n", "2", "-N", "8,8", "-h"], capture_output=True, text=True,
                        timeout=60, cwd=tmp_path)
     assert r.returncode == 0 and "13: u1_11" in r.stdout
+
+
+def test_error_codes_on_the_oracle(olib):
+    """Statuses of bad calls (the reference returns none; dp45.cu never checks)."""
+    import numpy as np
+    import pytest as _pt
+    from rdcu_amd import Integrator, Problem, RdcuError, Term, cgle_terms
+    from rdcu_amd.abi import EINVAL, ESTATE, EUNSUPPORTED
+
+    def code_of(fn):
+        with _pt.raises(RdcuError) as e:
+            fn()
+        return e.value.code
+
+    assert code_of(lambda: Integrator(olib, Problem(2, [31], [1.0], []), np.zeros(62))) == EUNSUPPORTED
+    bad = Problem(2, [16], [1.0], [Term(1.0, 0, [(99, 1)])])
+    assert code_of(lambda: Integrator(olib, bad, np.zeros(32))) == EINVAL
+    badf = Problem(2, [16], [1.0], [Term(1.0, 5, [(0, 1)])])
+    assert code_of(lambda: Integrator(olib, badf, np.zeros(32))) == EINVAL
+    p = Problem(2, [16], [8.0], cgle_terms(1))
+    with Integrator(olib, p, np.full(32, 0.1)) as it:
+        assert code_of(lambda: it.eval(0.5)) == ESTATE  # no accepted step yet
+        hs = np.full(4, 0.01)
+        tgt = np.zeros(32)
+        assert code_of(lambda: it.loss_grad(np.zeros(32), 0.0, hs, [1.0], tgt)) == EINVAL
+        assert code_of(lambda: it.loss_grad(np.zeros(32), 0.0, hs, [0.0], tgt)) == EINVAL
+        with _pt.raises(ValueError):
+            it.set_coef([1.0])

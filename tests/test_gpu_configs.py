@@ -96,3 +96,32 @@ def test_c3_full_size_bitwise(hlib, olib):
             to, ho, ao = o.step(to, ho)
             assert steps[i] == (to, ho, ao), i
         assert np.array_equal(yg, o.state())
+
+
+@pytest.mark.parametrize("Ns,Ls", [([2048], [400.0]), ([2048, 2048], [400.0, 400.0])],
+                         ids=["1d-2048", "2d-2048"])
+def test_largest_register_rows_bitwise(hlib, olib, Ns, Ls):
+    """The longest x lines the register path takes (2048 points, 128 threads
+    per line): RHS and two attempts bit for bit."""
+    prob = Problem(2, Ns, Ls, cgle_terms(len(Ns), 2.0, 0.8))
+    y0 = glibc_rand_ic(2, prob.N, seed=2, A=1.0)
+    print(f"case largest rows {Ns}", flush=True)
+    with Integrator(hlib, prob, y0) as g, Integrator(olib, prob, y0) as o:
+        assert np.array_equal(g.rhs(y0), o.rhs(y0))
+        tg = to = 0.0
+        hg = ho = 0.005
+        for i in range(2):
+            tg, hg, ag = g.step(tg, hg)
+            to, ho, ao = o.step(to, ho)
+            assert (tg, hg, ag) == (to, ho, ao)
+        assert np.array_equal(g.state(), o.state())
+
+
+def test_unsupported_sizes_fail_cleanly(hlib):
+    """Lengths no kernel takes are refused at creation (before any launch)."""
+    from rdcu_amd import RdcuError
+    for Ns in ([8192], [30, 31], [64, 4096]):
+        prob = Problem(2, Ns, [10.0] * len(Ns), cgle_terms(len(Ns)))
+        with pytest.raises(RdcuError) as e:
+            Integrator(hlib, prob, np.zeros(2 * prob.N))
+        assert e.value.code == 2  # RDCU_EUNSUPPORTED

@@ -45,6 +45,8 @@ RHS_CASES = [
     ("2d-96x384", [96, 384], [60.0, 240.0], "fhn"),
     ("2d-40x20-lds", [40, 20], [30.0, 15.0], "mixed"),    # radix-5 lengths: LDS kernels
     ("3d-10x12x8-lds", [10, 12, 8], [5.0, 6.0, 4.0], "cgle"),
+    ("2d-32x64-empty", [32, 64], [20.0, 30.0], "empty"),      # no terms: f = 0
+    ("2d-64x32-onefield", [64, 32], [30.0, 20.0], "onefield"),  # field 1 has no terms
 ]
 
 
@@ -54,6 +56,10 @@ def make_problem(Ns, Ls, kind):
         terms = cgle_terms(nd)
     elif kind == "fhn":
         terms = fhn_terms(nd, 4.0, 0.05, 1.2)
+    elif kind == "empty":
+        terms = []
+    elif kind == "onefield":
+        terms = [t for t in cgle_terms(nd) if t.field == 0]
     else:
         terms = mixed_terms(nd)
     return Problem(2, Ns, Ls, terms)
