@@ -150,6 +150,7 @@ struct rdcu_ctx {
   int reg = 0, Ex = 0, E0 = 0, E1 = 0;
   int rf_rpb = 1, cr_tc = 1, ri_G = 1, ri_R = 1, cr_fpw = 1, max_fpf = 1;
   size_t lds_rf = 0, lds_cr0 = 0, lds_cr1 = 0, lds_ri = 0;
+  bool lds_rhs = true, lds_y = true;  // the LDS kernels fit (see choose_shapes)
   std::string tune_log;
   // host-side flags and counters
   int fixed = 0;
@@ -874,20 +875,24 @@ int ensure_k1(rdcu_ctx *c) {
 
 // Launch shapes: rows per block and threads for the x passes, column tile
 // width for the strided passes, factor group size for the inverse x pass.
+// Shapes of the LDS-resident kernels.  lds_rhs: they can evaluate the RHS
+// (needed when the register path does not apply); lds_y: the all-factor
+// output of rdcu_rhs (strided pass + C2R store mode) fits.
 int choose_shapes(rdcu_ctx *c) {
   const int L = c->L, Nx = c->Nx, n = c->n;
   auto round64 = [](long v) { return (int)std::max<long>(64, std::min<long>(256, (v + 63) / 64 * 64)); };
+  c->lds_rhs = true;
+  c->lds_y = true;
   c->r2c_rpb = std::max(1, std::min(8, 256 / std::max(1, L)));
   c->r2c_thr = round64((long)c->r2c_rpb * L / 4);
   c->lds_r2c = (size_t)2 * c->r2c_rpb * L * sizeof(cx);
-  if (c->lds_r2c > kLdsMax) return fail(RDCU_EUNSUPPORTED, "x axis too long");
+  if (c->lds_r2c > kLdsMax) c->lds_rhs = false;
   if (c->ndim >= 2) {
     int Lmax = c->Ns[0];
     if (c->ndim == 3) Lmax = std::max(Lmax, c->Ns[1]);
     int tc = 1;
     while (tc * 2 <= 16 && (size_t)3 * (tc * 2) * Lmax * sizeof(cx) <= 48 * 1024) tc *= 2;
-    if ((size_t)3 * tc * Lmax * sizeof(cx) > kLdsMax)
-      return fail(RDCU_EUNSUPPORTED, "strided axis too long");
+    if ((size_t)3 * tc * Lmax * sizeof(cx) > kLdsMax) c->lds_rhs = c->lds_y = false;
     c->col_tc = tc;
     c->col_thr = round64((long)tc * Lmax / 4);
     c->lds_col = (size_t)3 * tc * Lmax * sizeof(cx);
@@ -899,14 +904,14 @@ int choose_shapes(rdcu_ctx *c) {
   };
   int grp = std::max(1, c->nf_used);
   while (grp > 1 && c2r_lds(c->nf_used, grp) > kLdsMax) --grp;
-  if (c2r_lds(c->nf_used, grp) > kLdsMax)
-    return fail(RDCU_EUNSUPPORTED, "x axis too long for the number of factors");
+  if (c2r_lds(c->nf_used, grp) > kLdsMax) c->lds_rhs = false;
   c->c2r_grp = grp;
   c->lds_c2r = c2r_lds(c->nf_used, grp);
   c->c2r_thr = round64((long)Nx / 2);
   // store mode keeps no value rows: only the FFT scratch of its groups
   int ga = std::max(1, c->nf_all);
   while (ga > 1 && (size_t)2 * ga * L * sizeof(cx) > kLdsMax) --ga;
+  if ((size_t)2 * ga * L * sizeof(cx) > kLdsMax) c->lds_y = false;
   c->c2r_grp_all = ga;
   c->lds_c2r_all = (size_t)2 * ga * L * sizeof(cx);
   return RDCU_OK;
@@ -1137,6 +1142,9 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
     const char *colp = getenv("RDCU_COL");
     if (colp && !strcmp(colp, "lds")) c->col_lds = 1;
     c->ri_G = c->n;
+    if (!c->reg && !c->lds_rhs)
+      return fail(RDCU_EUNSUPPORTED, "grid lines too long for the kernels of this build");
+    if (c->col_lds && !c->lds_y) return fail(RDCU_EUNSUPPORTED, "strided axis too long");
   }
 
   // buffers
@@ -1403,6 +1411,7 @@ int rdcu_rhs(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
   int st = enqueue_rhs(c, 0, rd::SRC_PTR, c->rhs_in, rd::DST_PTR, 0, c->rhs_out, 0, 0);
   if (st) return st;
   if (Y_out) {
+    if (!c->lds_y) return fail(RDCU_EUNSUPPORTED, "factor output not available for lines this long");
     if (!c->Yreal && (st = c->dalloc(&c->Yreal, (size_t)std::max(1, c->nf_all) * N))) return st;
     if ((st = enqueue_all_factors(c))) return st;
   }
