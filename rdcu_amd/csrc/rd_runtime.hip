@@ -1128,8 +1128,8 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
     bool ok = c->Ex > 0 && (c->ndim < 2 || c->E0 > 0) && (c->ndim < 3 || c->E1 > 0);
     const int T = c->Ex ? c->L / c->Ex : 0;
     ok = ok && c->n * T <= 256;  // one block of <= 256 threads per row position
-    ok = ok && (size_t)(c->L + c->n * rd::padded_len(c->L)) * sizeof(cx) +
-                       (size_t)(2 * c->n + c->nf_used) * c->Nx * 8 + (size_t)c->dictp.nwords * 4 <=
+    ok = ok && (size_t)rd::rowi_region(c->L, c->Nx, c->n, c->nf_used, c->dictp.nwords, c->n) *
+                       sizeof(cx) <=
                    160 * 1024;
     ok = ok && c->dictk_ok;
     if (c->ndim >= 2) ok = ok && c->Ns[0] / c->E0 <= 256 && (size_t)3 * c->Ns[0] * sizeof(cx) <= 150 * 1024;
