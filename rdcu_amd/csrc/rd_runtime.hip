@@ -396,6 +396,7 @@ void rowf(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, rd::Stage
 void colr(rdcu_ctx *c, const rd::FftPlan &p, int E, rd::ColGeom cg, long ngroups, int check,
           const rd::FacList &fl, int mode) {
   if (E == 8) launch_colr<8>(c, p, cg, ngroups, check, fl, mode);
+  else if (E == 16) launch_colr<16>(c, p, cg, ngroups, check, fl, mode);
   else launch_colr<24>(c, p, cg, ngroups, check, fl, mode);
 }
 void rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, int dst_kind, int dst_slot,
@@ -654,9 +655,15 @@ void build_jit(rdcu_ctx *c) {
 // must see only the launches of one shape), checked against the same limits.
 
 int fixed_shapes(rdcu_ctx *c, const char *spec) {
-  int rpb = 0, tc = 0, fpw = 0, G = 0, R = 1;
-  const int got = sscanf(spec, "%d,%d,%d,%d,%d", &rpb, &tc, &fpw, &G, &R);
-  if (got != 4 && got != 5) return fail(RDCU_EINVAL, "RDCU_SHAPES must be rpb,tc,fpw,G[,R]");
+  int rpb = 0, tc = 0, fpw = 0, G = 0, R = 1, Ec = 0;
+  const int got = sscanf(spec, "%d,%d,%d,%d,%d,%d", &rpb, &tc, &fpw, &G, &R, &Ec);
+  if (got < 4) return fail(RDCU_EINVAL, "RDCU_SHAPES must be rpb,tc,fpw,G[,R[,Ecol]]");
+  if (Ec == 16) {  // 16 values per thread on the power-of-two strided axes
+    if (c->E0 == 8 && c->Ns[0] >= 16) c->E0 = 16;
+    if (c->ndim == 3 && c->E1 == 8 && c->Ns[1] >= 16) c->E1 = 16;
+  } else if (Ec != 0 && Ec != 8) {
+    return fail(RDCU_EINVAL, "RDCU_SHAPES column values per thread must be 8 or 16");
+  }
   const int T = c->L / c->Ex;
   int Lmax = c->Ns[0], Emin = c->E0;
   if (c->ndim == 3) {
@@ -714,34 +721,48 @@ int autotune(rdcu_ctx *c) {
       c->rf_rpb = rpb;
     }
   }
-  // strided passes: column tile width
+  // strided passes: values per thread (8 or 16 for power-of-two lengths:
+  // the same butterflies either way), column tile width, factors per block
   if (c->ndim >= 2) {
     best = 1e30;
-    int Lmax = c->Ns[0];
-    int Emin = c->E0;
-    if (c->ndim == 3) {
-      Lmax = std::max(Lmax, c->Ns[1]);
-      Emin = std::min(c->E0, c->E1);
-    }
+    const int E0b = c->E0, E1b = c->E1;
+    int bestE0 = E0b, bestE1 = E1b;
     int best_fpw = c->max_fpf;
-    for (int tc = std::max(1, 64 / (Lmax / Emin)); tc <= 64; tc *= 2) {
-      if (tc * (Lmax / Emin) > 256 ||
-          (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) > 160 * 1024)
-        break;
-      for (int fpw = c->max_fpf; fpw >= 1; fpw = fpw > 1 ? (fpw + 1) / 2 : 0) {
-        c->cr_fpw = fpw;
-        auto ls = col_launches_reg(c, 0, tc);
-        double tm = time_of([&]() { for (auto &l : ls) l.go(); });
-        snprintf(line, sizeof line, "col tc=%d fpw=%d %.2fus; ", tc, fpw, tm * 1e3);
-        c->tune_log += line;
-        if (tm < best) {
-          best = tm;
-          c->cr_tc = tc;
-          best_fpw = fpw;
+    for (int ev = 0; ev < 2; ++ev) {
+      if (ev == 1) {
+        bool any = false;
+        if (E0b == 8 && c->Ns[0] >= 16) c->E0 = 16, any = true;
+        if (c->ndim == 3 && E1b == 8 && c->Ns[1] >= 16) c->E1 = 16, any = true;
+        if (!any) break;
+      }
+      int Lmax = c->Ns[0], Tmax = c->Ns[0] / c->E0;
+      if (c->ndim == 3) {
+        Lmax = std::max(Lmax, c->Ns[1]);
+        Tmax = std::max(Tmax, c->Ns[1] / c->E1);
+      }
+      for (int tc = std::max(1, 64 / Tmax); tc <= 64; tc *= 2) {
+        if (tc * Tmax > 256 ||
+            (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) > 160 * 1024)
+          break;
+        for (int fpw = c->max_fpf; fpw >= 1; fpw = fpw > 1 ? (fpw + 1) / 2 : 0) {
+          c->cr_fpw = fpw;
+          auto ls = col_launches_reg(c, 0, tc);
+          double tm = time_of([&]() { for (auto &l : ls) l.go(); });
+          snprintf(line, sizeof line, "col E=%d tc=%d fpw=%d %.2fus; ", c->E0, tc, fpw, tm * 1e3);
+          c->tune_log += line;
+          if (tm < best) {
+            best = tm;
+            c->cr_tc = tc;
+            best_fpw = fpw;
+            bestE0 = c->E0;
+            bestE1 = c->E1;
+          }
+          if (fpw == 1) break;
         }
-        if (fpw == 1) break;
       }
     }
+    c->E0 = bestE0;
+    c->E1 = bestE1;
     c->cr_fpw = best_fpw;
   }
   // x inverse: factor lines per group, rows per block (dividing the rows)
@@ -778,6 +799,7 @@ int allow_big_lds() {
   HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowf_reg<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   HIPCHECK(hipFuncSetAttribute((const void *)rd::k_col_reg<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   HIPCHECK(hipFuncSetAttribute((const void *)rd::k_col_reg<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_col_reg<16>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_reg<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_reg<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_vjp<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
@@ -1215,9 +1237,9 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
     }
     const char *vb = getenv("RDCU_VERBOSE");
     if (vb && *vb && *vb != '0')
-      fprintf(stderr, "rdcu jit: %s\nrdcu autotune: %s\nrdcu shapes: %d,%d,%d,%d,%d\n",
+      fprintf(stderr, "rdcu jit: %s\nrdcu autotune: %s\nrdcu shapes: %d,%d,%d,%d,%d,%d\n",
               c->jit_log.c_str(), c->tune_log.c_str(), c->rf_rpb, c->cr_tc, c->cr_fpw, c->ri_G,
-              c->ri_R);
+              c->ri_R, c->E0 == 16 ? 16 : 8);
   }
 
   // capture one attempt

@@ -208,3 +208,25 @@ def test_cli_outputs_identical(tmp_path):
         a = (tmp_path / "g" / ("run" + suf)).read_bytes()
         b = (tmp_path / "c" / ("run" + suf)).read_bytes()
         assert a == b, suf
+
+
+@pytest.mark.parametrize("Ns,Ls,shapes", [([512, 512], [200.0, 200.0], "2,8,1,4,1,16"),
+                                          ([64, 128], [50.0, 90.0], "4,4,2,2,2,16"),
+                                          ([32, 32, 32], [30.0, 30.0, 30.0], "2,8,1,4,1,16")],
+                         ids=["2d-512", "2d-64x128", "3d-32"])
+def test_sixteen_values_per_thread_columns_bitwise(hlib, olib, monkeypatch, Ns, Ls, shapes):
+    """Strided passes with 16 values per thread (the same butterflies as with
+    8) give the oracle's bits; shapes forced so the case is always covered."""
+    prob = make_problem(Ns, Ls, "cgle")
+    y0 = glibc_rand_ic(prob.n, prob.N, seed=6, A=1.0)
+    monkeypatch.setenv("RDCU_SHAPES", shapes)
+    print(f"case E16 {Ns}", flush=True)
+    with Integrator(hlib, prob, y0) as g, Integrator(olib, prob, y0) as o:
+        assert np.array_equal(g.rhs(y0), o.rhs(y0))
+        tg = to = 0.0
+        hg = ho = 0.01
+        for i in range(3):
+            tg, hg, ag = g.step(tg, hg)
+            to, ho, ao = o.step(to, ho)
+            assert (tg, hg, ag) == (to, ho, ao)
+        assert np.array_equal(g.state(), o.state())
