@@ -14,6 +14,18 @@
 
 namespace rd {
 
+// Streaming stores for arrays the next kernel reads from HBM: written
+// through rather than held dirty in the XCD's L2 until the end-of-kernel
+// write-back.
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_nt(cx *p, cx v) {
+  __builtin_nontemporal_store(nt_d2{v.re, v.im}, reinterpret_cast<nt_d2 *>(p));
+}
+__device__ __forceinline__ void st_nt(double2 *p, double2 v) {
+  __builtin_nontemporal_store(nt_d2{v.x, v.y}, reinterpret_cast<nt_d2 *>(p));
+}
+__device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+
 // Field f of the dictionary at XM points x0 + stride*i of one row (value rows
 // in LDS, vals[slot*Nx + x]): terms of field f in input order, each a product
 // of integer powers started from 1, accumulated with fma from 0 -- the
@@ -187,7 +199,7 @@ __device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan
         double2 o;
         o.x = __builtin_fma(h, ar, yv.x);
         o.y = __builtin_fma(h, ai, yv.y);
-        *reinterpret_cast<double2 *>(yt + off) = o;
+        st_nt(reinterpret_cast<double2 *>(yt + off), o);
         v[q] = cx{o.x, o.y};
       }
     } else {
@@ -218,14 +230,14 @@ __device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan
     const cx Bv = cconj(lb[k == 0 ? 0 : L - k]);
     const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
     const cx tt = cmul(D, twN[k]);
-    S[k] = cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)};
+    st_nt(S + k, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
   }
   if (t == 0) {
     const cx Av = lb[0];
     const cx Bv = cconj(lb[0]);
     const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
     const cx tt = cmul(D, twN[L]);
-    S[L] = cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)};
+    st_nt(S + L, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
   }
 }
 
@@ -305,7 +317,7 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     else reg_fft<E, +1>(v, lb, tc, tw, p, t, T);
     if (act) {
 #pragma unroll
-      for (int q = 0; q < E; ++q) src[(long)(t + T * q) * cg.es] = v[q];
+      for (int q = 0; q < E; ++q) st_nt(src + (long)(t + T * q) * cg.es, v[q]);
     }
     return;
   }
@@ -328,7 +340,7 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     if (act) {
       cx *dst = b.F + (long)m * g.Nh + lbase;
 #pragma unroll
-      for (int q = 0; q < E; ++q) dst[(long)(t + T * q) * cg.es] = w[q];
+      for (int q = 0; q < E; ++q) st_nt(dst + (long)(t + T * q) * cg.es, w[q]);
     }
   }
 }
@@ -500,7 +512,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
         const int x = x0 + nt * i;
         if (x >= Nx) continue;
         const long idx = f * N + r * Nx + x;
-        out[idx] = acc[i];
+        st_nt(out + idx, acc[i]);
         if (nx.ns) fv[f * Nx + x] = acc[i];
         if (err) {
           double e = ec.e[0] * ev[i][0];
@@ -570,7 +582,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
       double2 o;
       o.x = __builtin_fma(hs, ar, yv2.x);
       o.y = __builtin_fma(hs, ai, yv2.y);
-      *reinterpret_cast<double2 *>(yt + off) = o;
+      st_nt(reinterpret_cast<double2 *>(yt + off), o);
       *reinterpret_cast<double2 *>(ys + f * Nx + j2) = o;
     }
   }
@@ -596,14 +608,14 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
     const cx Bv = cconj(lb[k == 0 ? 0 : L - k]);
     const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
     const cx tt = cmul(D, twN[k]);
-    S[k] = cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)};
+    st_nt(S + k, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
   }
   if (t == 0) {
     const cx Av = lb[0];
     const cx Bv = cconj(lb[0]);
     const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
     const cx tt = cmul(D, twN[L]);
-    S[L] = cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)};
+    st_nt(S + L, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
   }
 }
 
