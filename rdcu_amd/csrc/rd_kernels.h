@@ -62,6 +62,7 @@ struct Grid {
   int Ns[3];
   int Nx, Nxh, L;
   int Nxp;  // row pitch of half spectra (complex): Nxh rounded up to 8 (128 B)
+  int nt_store;  // streaming stores in the x-forward and strided passes
   long N, rows, Nh;  // Nh = rows * Nxp
   double invN;
   const double *kt[3];  // wavenumbers per axis (device)

@@ -16,7 +16,8 @@ namespace rd {
 
 // Streaming stores for arrays the next kernel reads from HBM: written
 // through rather than held dirty in the XCD's L2 until the end-of-kernel
-// write-back.
+// write-back.  Used by the x-forward and strided passes of large problems
+// (Grid::nt_store); measured slower on problems that fit in the caches.
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void st_nt(cx *p, cx v) {
   __builtin_nontemporal_store(nt_d2{v.re, v.im}, reinterpret_cast<nt_d2 *>(p));
@@ -25,6 +26,12 @@ __device__ __forceinline__ void st_nt(double2 *p, double2 v) {
   __builtin_nontemporal_store(nt_d2{v.x, v.y}, reinterpret_cast<nt_d2 *>(p));
 }
 __device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+// streaming or ordinary store, by the grid's choice (uniform branch)
+template <class T>
+__device__ __forceinline__ void st_g(const Grid &g, T *p, T v) {
+  if (g.nt_store) st_nt(p, v);
+  else *p = v;
+}
 
 // Field f of the dictionary at XM points x0 + stride*i of one row (value rows
 // in LDS, vals[slot*Nx + x]): terms of field f in input order, each a product
@@ -199,7 +206,7 @@ __device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan
         double2 o;
         o.x = __builtin_fma(h, ar, yv.x);
         o.y = __builtin_fma(h, ai, yv.y);
-        st_nt(reinterpret_cast<double2 *>(yt + off), o);
+        st_g(g, reinterpret_cast<double2 *>(yt + off), o);
         v[q] = cx{o.x, o.y};
       }
     } else {
@@ -230,14 +237,14 @@ __device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan
     const cx Bv = cconj(lb[k == 0 ? 0 : L - k]);
     const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
     const cx tt = cmul(D, twN[k]);
-    st_nt(S + k, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
+    st_g(g, S + k, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
   }
   if (t == 0) {
     const cx Av = lb[0];
     const cx Bv = cconj(lb[0]);
     const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
     const cx tt = cmul(D, twN[L]);
-    st_nt(S + L, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
+    st_g(g, S + L, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
   }
 }
 
@@ -317,7 +324,7 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     else reg_fft<E, +1>(v, lb, tc, tw, p, t, T);
     if (act) {
 #pragma unroll
-      for (int q = 0; q < E; ++q) st_nt(src + (long)(t + T * q) * cg.es, v[q]);
+      for (int q = 0; q < E; ++q) st_g(g, src + (long)(t + T * q) * cg.es, v[q]);
     }
     return;
   }
@@ -340,7 +347,7 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     if (act) {
       cx *dst = b.F + (long)m * g.Nh + lbase;
 #pragma unroll
-      for (int q = 0; q < E; ++q) st_nt(dst + (long)(t + T * q) * cg.es, w[q]);
+      for (int q = 0; q < E; ++q) st_g(g, dst + (long)(t + T * q) * cg.es, w[q]);
     }
   }
 }
@@ -512,7 +519,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
         const int x = x0 + nt * i;
         if (x >= Nx) continue;
         const long idx = f * N + r * Nx + x;
-        st_nt(out + idx, acc[i]);
+        out[idx] = acc[i];
         if (nx.ns) fv[f * Nx + x] = acc[i];
         if (err) {
           double e = ec.e[0] * ev[i][0];
@@ -582,7 +589,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
       double2 o;
       o.x = __builtin_fma(hs, ar, yv2.x);
       o.y = __builtin_fma(hs, ai, yv2.y);
-      st_nt(reinterpret_cast<double2 *>(yt + off), o);
+      *reinterpret_cast<double2 *>(yt + off) = o;
       *reinterpret_cast<double2 *>(ys + f * Nx + j2) = o;
     }
   }
@@ -608,14 +615,14 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
     const cx Bv = cconj(lb[k == 0 ? 0 : L - k]);
     const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
     const cx tt = cmul(D, twN[k]);
-    st_nt(S + k, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
+    S[k] = cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)};
   }
   if (t == 0) {
     const cx Av = lb[0];
     const cx Bv = cconj(lb[0]);
     const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
     const cx tt = cmul(D, twN[L]);
-    st_nt(S + L, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
+    S[L] = cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)};
   }
 }
 

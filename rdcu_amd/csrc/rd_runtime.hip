@@ -1075,6 +1075,8 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
   g.Nx = c->Nx;
   g.Nxh = c->Nxh;
   g.Nxp = c->Nxp;
+  // streaming stores once a state vector no longer fits the 32 MB of L2
+  g.nt_store = (double)c->n * c->N * sizeof(double) >= 32.0 * (1 << 20) ? 1 : 0;
   g.L = c->L;
   g.N = c->N;
   g.rows = c->rows;
