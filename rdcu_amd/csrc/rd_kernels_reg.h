@@ -26,10 +26,10 @@ __device__ __forceinline__ void st_nt(double2 *p, double2 v) {
   __builtin_nontemporal_store(nt_d2{v.x, v.y}, reinterpret_cast<nt_d2 *>(p));
 }
 __device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
-// streaming or ordinary store, by the grid's choice (uniform branch)
-template <class T>
-__device__ __forceinline__ void st_g(const Grid &g, T *p, T v) {
-  if (g.nt_store) st_nt(p, v);
+// streaming (NT) or ordinary store, chosen at compile time
+template <bool NT, class T>
+__device__ __forceinline__ void st_g(T *p, T v) {
+  if constexpr (NT) st_nt(p, v);
   else *p = v;
 }
 
@@ -166,7 +166,7 @@ __device__ __forceinline__ cx mult_k(const Grid &g, int blk, const int *nk, cons
 // Forward x pass of rows (plain input or DP45 stage sum) -> S.
 // Block: rpb rows x T threads (T = L/E).  LDS: twiddles L + rpb*padded_len(L)
 // complex.
-template <int E>
+template <int E, bool NT = false>
 __device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
                                           int src_kind, const double *src_ptr, const StageCoef &sc,
@@ -206,7 +206,7 @@ __device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan
         double2 o;
         o.x = __builtin_fma(h, ar, yv.x);
         o.y = __builtin_fma(h, ai, yv.y);
-        st_g(g, reinterpret_cast<double2 *>(yt + off), o);
+        st_g<NT>(reinterpret_cast<double2 *>(yt + off), o);
         v[q] = cx{o.x, o.y};
       }
     } else {
@@ -237,30 +237,30 @@ __device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan
     const cx Bv = cconj(lb[k == 0 ? 0 : L - k]);
     const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
     const cx tt = cmul(D, twN[k]);
-    st_g(g, S + k, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
+    st_g<NT>(S + k, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
   }
   if (t == 0) {
     const cx Av = lb[0];
     const cx Bv = cconj(lb[0]);
     const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
     const cx tt = cmul(D, twN[L]);
-    st_g(g, S + L, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
+    st_g<NT>(S + L, cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)});
   }
 }
 
-template <int E>
+template <int E, bool NT>
 __global__ __launch_bounds__(256) void k_rowf_reg(Grid g, FftPlan px, const cx *__restrict__ twN,
                                                   DevState *st, int check_active, Bufs b,
                                                   int src_kind, const double *src_ptr,
                                                   StageCoef sc, int rpb) {
   if (check_active && !attempt_active(st)) return;
-  rowf_body<E>(blockIdx.x, g, px, twN, st, b, src_kind, src_ptr, sc, rpb);
+  rowf_body<E, NT>(blockIdx.x, g, px, twN, st, b, src_kind, src_ptr, sc, rpb);
 }
 
 // Strided-line pass (modes as k_col).  Block: tc columns x T threads; LDS:
 // twiddles L + tc*padded_len(L) complex (element e of column c at
 // buf[pad8(e)*tc + c]) + tc*L complex holding the forward spectrum (mode 1).
-template <int E>
+template <int E, bool NT = false>
 __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan &p,
                                          const ColGeom &cg, const Bufs &b, const FacList &fl,
                                          int mode, int fpw) {
@@ -324,7 +324,7 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     else reg_fft<E, +1>(v, lb, tc, tw, p, t, T);
     if (act) {
 #pragma unroll
-      for (int q = 0; q < E; ++q) st_g(g, src + (long)(t + T * q) * cg.es, v[q]);
+      for (int q = 0; q < E; ++q) st_g<NT>(src + (long)(t + T * q) * cg.es, v[q]);
     }
     return;
   }
@@ -347,19 +347,19 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     if (act) {
       cx *dst = b.F + (long)m * g.Nh + lbase;
 #pragma unroll
-      for (int q = 0; q < E; ++q) st_g(g, dst + (long)(t + T * q) * cg.es, w[q]);
+      for (int q = 0; q < E; ++q) st_g<NT>(dst + (long)(t + T * q) * cg.es, w[q]);
     }
   }
 }
 
-template <int E>
+template <int E, bool NT>
 __global__ __launch_bounds__(256) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
                                                  int check_active, Bufs b, FacList fl, int mode,
                                                  int fpw) {
   if (check_active && !attempt_active(st)) return;
   const int lblk = xcd_block(blockIdx.x, gridDim.x);
   if (lblk >= cg.nblk) return;
-  col_body<E>(lblk, g, p, cg, b, fl, mode, fpw);
+  col_body<E, NT>(lblk, g, p, cg, b, fl, mode, fpw);
 }
 
 // Inverse x pass for one row position r, all fields: factor lines in groups

@@ -328,8 +328,12 @@ void launch_rowf(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, rd
   const int T = c->L / E;
   dim3 g((unsigned)((nrows + rpb - 1) / rpb));
   size_t lds = (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx);
-  hipLaunchKernelGGL(rd::k_rowf_reg<E>, g, dim3(rpb * T), lds, c->stream, c->grid, c->px, c->twN,
-                     c->st, check, c->bufs, src_kind, src_ptr, sc, rpb);
+  if (c->grid.nt_store)
+    hipLaunchKernelGGL((rd::k_rowf_reg<E, true>), g, dim3(rpb * T), lds, c->stream, c->grid, c->px,
+                       c->twN, c->st, check, c->bufs, src_kind, src_ptr, sc, rpb);
+  else
+    hipLaunchKernelGGL((rd::k_rowf_reg<E, false>), g, dim3(rpb * T), lds, c->stream, c->grid, c->px,
+                       c->twN, c->st, check, c->bufs, src_kind, src_ptr, sc, rpb);
 }
 
 template <int E>
@@ -344,8 +348,12 @@ void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups
   size_t lds = (size_t)(cg.L + cg.tc * rd::padded_len(cg.L) + (mode == 1 ? cg.tc * cg.L : 0)) *
                    sizeof(cx) +
                (mode == 1 ? (size_t)cg.L * sizeof(double) : 0);
-  hipLaunchKernelGGL(rd::k_col_reg<E>, g, dim3(cg.tc * T), lds, c->stream, c->grid, p, cg, c->st,
-                     check, c->bufs, fl, mode, fpw);
+  if (c->grid.nt_store)
+    hipLaunchKernelGGL((rd::k_col_reg<E, true>), g, dim3(cg.tc * T), lds, c->stream, c->grid, p, cg,
+                       c->st, check, c->bufs, fl, mode, fpw);
+  else
+    hipLaunchKernelGGL((rd::k_col_reg<E, false>), g, dim3(cg.tc * T), lds, c->stream, c->grid, p, cg,
+                       c->st, check, c->bufs, fl, mode, fpw);
 }
 
 size_t rowi_lds(const rdcu_ctx *c, int G, int R) {
@@ -795,15 +803,16 @@ int autotune(rdcu_ctx *c) {
 // a gfx950 CU.
 int allow_big_lds() {
   const int lim = 160 * 1024;
-  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowf_reg<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowf_reg<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_col_reg<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_col_reg<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_col_reg<16>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_reg<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_reg<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_vjp<8>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  HIPCHECK(hipFuncSetAttribute((const void *)rd::k_rowi_vjp<24>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  const void *fs[] = {
+      (const void *)rd::k_rowf_reg<8, false>,  (const void *)rd::k_rowf_reg<8, true>,
+      (const void *)rd::k_rowf_reg<24, false>, (const void *)rd::k_rowf_reg<24, true>,
+      (const void *)rd::k_col_reg<8, false>,   (const void *)rd::k_col_reg<8, true>,
+      (const void *)rd::k_col_reg<16, false>,  (const void *)rd::k_col_reg<16, true>,
+      (const void *)rd::k_col_reg<24, false>,  (const void *)rd::k_col_reg<24, true>,
+      (const void *)rd::k_rowi_reg<8>,         (const void *)rd::k_rowi_reg<24>,
+      (const void *)rd::k_rowi_vjp<8>,         (const void *)rd::k_rowi_vjp<24>};
+  for (const void *f : fs)
+    HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return RDCU_OK;
 }
 
