@@ -27,7 +27,7 @@ struct BatchClass {
   rdcu_ctx *c0 = nullptr;    // a representative member (geometry, plans, tables)
   int Ex = 0, E0 = 0, rows = 0, nf_max = 0, nwords_max = 0, max_fpf = 1;
   int rpb = 1, tc = 1, fpw = 1, G = 1, R = 1;
-  long grid_rf = 0, grid_col = 0;  // persistent grid sizes (0: one block per item)
+  long grid_rf = 0;  // persistent grid of the x-forward pass (0: one block per item)
   std::string tune_log;
 };
 
@@ -96,7 +96,7 @@ void launch_rowf_b(rdcu_batch *b, const BatchClass &cl, int src_kind, rd::StageC
 }
 
 template <int E>
-void launch_col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, long grid, int check) {
+void launch_col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, int check) {
   const rdcu_ctx *c = cl.c0;
   rd::ColGeom cg{};
   cg.L = c->Ns[0];
@@ -109,7 +109,6 @@ void launch_col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, long gri
   const int per = c->n * cg.ntc * cg.nparts;
   cg.nblk = per;
   const int T = cg.L / E;
-  (void)grid;
   const long total = (long)cl.cnt * per;
   if (total == 0) return;
   const long g = (total + 7) / 8 * 8;  // one item per block, a multiple of the 8 XCDs
@@ -155,10 +154,10 @@ void rowf_b(rdcu_batch *b, const BatchClass &cl, int src_kind, rd::StageCoef sc,
   if (cl.Ex == 8) launch_rowf_b<8>(b, cl, src_kind, sc, rpb, grid, check);
   else launch_rowf_b<24>(b, cl, src_kind, sc, rpb, grid, check);
 }
-void col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, long grid, int check) {
+void col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, int check) {
   if (cl.c0->ndim < 2) return;
-  if (cl.E0 == 8) launch_col_b<8>(b, cl, tc, fpw, grid, check);
-  else launch_col_b<24>(b, cl, tc, fpw, grid, check);
+  if (cl.E0 == 8) launch_col_b<8>(b, cl, tc, fpw, check);
+  else launch_col_b<24>(b, cl, tc, fpw, check);
 }
 void rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_kind, int dst_slot, int err,
             rd::StageCoef nx, int G, int R, int check) {
@@ -171,7 +170,7 @@ void rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_kind, int
 void rhs_b(rdcu_batch *b, const BatchClass &cl, int dst_kind, int slot, int check) {
   rd::StageCoef none{};
   rowf_b(b, cl, rd::SRC_Y, none, cl.rpb, cl.grid_rf, check);
-  col_b(b, cl, cl.tc, cl.fpw, cl.grid_col, check);
+  col_b(b, cl, cl.tc, cl.fpw, check);
   rowi_b(b, cl, rd::SRC_Y, dst_kind, slot, 0, none, cl.G, cl.R, check);
 }
 
@@ -181,7 +180,7 @@ int enqueue_attempt_b(rdcu_batch *b) {
   for (const BatchClass &cl : b->classes) {
     rowf_b(b, cl, rd::SRC_STAGE, stage_coef(2), cl.rpb, cl.grid_rf, 1);
     for (int s = 2; s <= 7; ++s) {
-      col_b(b, cl, cl.tc, cl.fpw, cl.grid_col, 1);
+      col_b(b, cl, cl.tc, cl.fpw, 1);
       rd::StageCoef nx = s < 7 ? stage_coef(s + 1) : rd::StageCoef{};
       const int dk = s < 7 ? rd::DST_KS : rd::DST_K7;
       rowi_b(b, cl, rd::SRC_YT, dk, s - 1, s == 7 && !b->fixed, nx, cl.G, cl.R, 1);
@@ -240,7 +239,7 @@ int autotune_b(rdcu_batch *b, BatchClass &cl) {
               160 * 1024)
         break;
       for (int fpw = cl.max_fpf; fpw >= 1; fpw = fpw > 1 ? (fpw + 1) / 2 : 0) {
-        double tm = time_of([&]() { col_b(b, cl, tc, fpw, 0, 0); });
+        double tm = time_of([&]() { col_b(b, cl, tc, fpw, 0); });
         snprintf(line, sizeof line, "col tc=%d fpw=%d %.1fus; ", tc, fpw, tm * 1e3);
         cl.tune_log += line;
         if (tm < best) {
@@ -588,10 +587,10 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
     const char *vb = getenv("RDCU_VERBOSE");
     if (vb && *vb && *vb != '0')
       fprintf(stderr, "rdcu batch class %dx%d (%zu instances, %zu structures, jit %s): %s -> "
-                      "%d,%d,%d,%d,%d grids %ld,%ld\n",
+                      "%d,%d,%d,%d,%d x-forward grid %ld\n",
               cl.c0->Ns[0], cl.c0->Ns[1], cl.members.size(), cl.groups.size(),
-              b->jit_log.c_str(), cl.tune_log.c_str(), cl.rpb, cl.tc, cl.fpw, cl.G, cl.R, cl.grid_rf,
-              cl.grid_col);
+              b->jit_log.c_str(), cl.tune_log.c_str(), cl.rpb, cl.tc, cl.fpw, cl.G, cl.R,
+              cl.grid_rf);
   }
   if ((st = capture_b(b.get()))) return st;
   HIPCHECK(hipStreamSynchronize(b->stream));
