@@ -222,7 +222,7 @@ __global__ void k_row_r2c(Grid g, FftPlan px, const cx *__restrict__ twN, DevSta
                           StageCoef sc, int rpb) {
   extern __shared__ cx smem[];
   if (check_active && !attempt_active(st)) return;
-  const int L = g.L, Nx = g.Nx, Nxh = g.Nxh;
+  const int L = g.L, Nx = g.Nx;
   const long nrows = (long)g.n * g.rows;
   const long row0 = (long)blockIdx.x * rpb;
   const int nr = (int)((nrows - row0) < rpb ? (nrows - row0) : rpb);
@@ -384,7 +384,7 @@ __global__ void k_row_c2r(Grid g, FftPlan px, const cx *__restrict__ twN, DevSta
                           double *dst_ptr, int err, ErrCoef ec, int grp, double *Yout) {
   extern __shared__ double dsm[];
   if (check_active && !attempt_active(st)) return;
-  const int n = g.n, Nx = g.Nx, L = g.L, Nxh = g.Nxh;
+  const int n = g.n, Nx = g.Nx, L = g.L;
   const long N = g.N;
   const long r = blockIdx.x;
   const int tid = threadIdx.x, nt = blockDim.x;

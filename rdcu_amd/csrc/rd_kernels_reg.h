@@ -172,7 +172,7 @@ __device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan
                                           int src_kind, const double *src_ptr, const StageCoef &sc,
                                           int rpb) {
   extern __shared__ cx smem[];
-  const int L = g.L, Nx = g.Nx, Nxh = g.Nxh, T = L / E;
+  const int L = g.L, Nx = g.Nx, T = L / E;
   const long nrows = (long)g.n * g.rows;
   cx *tw = smem, *buf = smem + L;
   const int tid = threadIdx.x;
@@ -413,7 +413,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
                                           const double *src_ptr, int dst_kind, int dst_slot,
                                           double *dst_ptr, int err, const ErrCoef &ec,
                                           const StageCoef &nx, int G, cx *smem, int tid, int nt) {
-  const int n = g.n, Nx = g.Nx, L = g.L, Nxh = g.Nxh, T = L / E;
+  const int n = g.n, Nx = g.Nx, L = g.L, T = L / E;
   const long N = g.N;
   const int gl = tid / T, t = tid - gl * T;
   const int nf = fl.nf;
