@@ -16,6 +16,7 @@
 //     (rdcu.cu:268 reads one past the end).
 //   * Integrator failures are reported and give exit status 1.
 //   * FILEBASEY.dat imaginary parts are 0.
+//   * FILEBASEfs.dat is replaced atomically (temporary file, then rename).
 
 #include <getopt.h>
 #include <sys/time.h>
@@ -109,6 +110,22 @@ void put(FILE *f, const void *p, size_t sz, size_t cnt) {
   }
 }
 
+// FILEBASEfs.dat = state, t, h (the layout -R reads back, rdcu.cu:322-328).
+// Written to a temporary name and renamed over the old file, so a run killed
+// mid-write leaves the previous restart point intact.
+void write_fs(const std::string &filebase, const double *y, size_t nN, double t, double h) {
+  const std::string fn = filebase + "fs.dat", tmp = fn + ".tmp";
+  FILE *fs = fopen(tmp.c_str(), "wb");
+  if (!fs) return;
+  bool ok = fwrite(y, sizeof(double), nN, fs) == nN;
+  ok = ok && fwrite(&t, sizeof(double), 1, fs) == 1;
+  ok = ok && fwrite(&h, sizeof(double), 1, fs) == 1;
+  ok = (fclose(fs) == 0) && ok;
+  if (ok && rename(tmp.c_str(), fn.c_str()) == 0) return;
+  fprintf(stderr, "could not write %s\n", fn.c_str());
+  remove(tmp.c_str());
+}
+
 // Per-attempt output, as step_eval (rdcu.cu:252-341).
 int step_eval(rdcu_ctx *ctx, double t, double h, int success, int current, double *stop,
               void *user) {
@@ -160,14 +177,7 @@ int step_eval(rdcu_ctx *ctx, double t, double h, int success, int current, doubl
       d.failed = 1;
       return 1;
     }
-    std::string fn = d.filebase + "fs.dat";
-    FILE *fs = fopen(fn.c_str(), "wb");
-    if (fs) {
-      fwrite(d.yloc.data(), sizeof(double), nN, fs);
-      fwrite(&t, sizeof(double), 1, fs);
-      fwrite(&h, sizeof(double), 1, fs);
-      fclose(fs);
-    }
+    write_fs(d.filebase, d.yloc.data(), nN, t, h);
   }
   return 0;
 }
@@ -478,13 +488,7 @@ int main(int argc, char *argv[]) {
     fprintf(d.out, "Using random initial conditions\n");
     for (int i = 0; i < n; i++)
       for (long j = 0; j < N; j++) yloc[N * i + j] = A0 + A / 2 * (2.0 / RAND_MAX * rand() - 1);
-    FILE *fs = fopen(file.c_str(), "wb");
-    if (fs) {
-      fwrite(yloc.data(), sizeof(double), nN, fs);
-      fwrite(&t, sizeof(double), 1, fs);
-      fwrite(&h, sizeof(double), 1, fs);
-      fclose(fs);
-    }
+    write_fs(d.filebase, yloc.data(), nN, t, h);
   }
   fflush(d.out);
   fflush(stdout);

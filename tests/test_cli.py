@@ -81,6 +81,8 @@ def test_output_files_layout(tmp_path):
     # f frame = RHS of the state frame; the identity block is the state
     fs = np.fromfile(tmp_path / "ofs.dat")
     assert fs.size == n * N + 2 and fs[-2] >= 0.5
+    # fs.dat goes through a temporary file that the rename consumes
+    assert not (tmp_path / "ofs.dat.tmp").exists()
 
 
 def test_restart_is_bitwise_continuation(tmp_path):
