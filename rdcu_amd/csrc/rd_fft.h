@@ -266,4 +266,88 @@ __device__ void reg_fft(cx (&v)[E], cx *buf, int es, const cx *tw, const FftPlan
   }
 }
 
+// ------------------------------------------------------------------------
+// The same transform with the length known at compile time.  The radix plan
+// (the host's order: 5s, 3s, one 2 or 4, 8s) is folded in, so every pass's
+// span, twiddle stride and index arithmetic are constants and the pass loop
+// unrolls; the arithmetic of each butterfly is that of reg_pass, so results
+// are identical to reg_fft's.
+constexpr unsigned long long static_plan_code(int L) {
+  int a = 0, b = 0, c = 0, m = L;
+  while (m % 5 == 0) { m /= 5; ++c; }
+  while (m % 3 == 0) { m /= 3; ++b; }
+  while (m % 2 == 0) { m /= 2; ++a; }
+  unsigned long long code = 0;
+  int np = 0;
+  for (int i = 0; i < c; ++i) code |= 5ull << (4 * np++);
+  for (int i = 0; i < b; ++i) code |= 3ull << (4 * np++);
+  if (a % 3 == 1) code |= 2ull << (4 * np++);
+  if (a % 3 == 2) code |= 4ull << (4 * np++);
+  for (int i = 0; i < a / 3; ++i) code |= 8ull << (4 * np++);
+  return code;
+}
+constexpr int static_plan_np(int L) {
+  int np = 0;
+  for (unsigned long long c = static_plan_code(L); c; c >>= 4) ++np;
+  return np;
+}
+constexpr int static_plan_rad(int L, int i) { return (int)((static_plan_code(L) >> (4 * i)) & 15u); }
+
+template <int R, int E, int DIR, int L, int NS, bool LAST>
+__device__ __forceinline__ void reg_pass_s(cx (&v)[E], cx *buf, int es, const cx *tw, int t) {
+  constexpr int B = E / R, T = L / E, TWS = L / (NS * R);
+  static_assert(E % R == 0, "values per thread must be a multiple of every radix");
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = t + T * b;
+    const int k = NS == 1 ? 0 : j % NS;
+    cx u[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) u[r] = v[b + B * r];
+    if (NS > 1 && k != 0) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cx w = tw[(k * r) * TWS];
+        if constexpr (DIR > 0) w.im = -w.im;
+        u[r] = cmul(u[r], w);
+      }
+    }
+    dft<R, DIR>(u);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + B * r] = u[r];
+    } else {
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[pad8(base + r * NS) * es] = u[r];
+    }
+  }
+}
+
+template <int E, int DIR, int L, int IP = 0, int NS = 1>
+__device__ __forceinline__ void reg_fft_s(cx (&v)[E], cx *buf, int es, const cx *tw, int t) {
+  constexpr int NP = static_plan_np(L);
+  if constexpr (IP < NP) {
+    constexpr int R = static_plan_rad(L, IP);
+    constexpr int T = L / E;
+    if constexpr (IP > 0) {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = buf[pad8(t + T * q) * es];
+      __syncthreads();
+    }
+    reg_pass_s<R, E, DIR, L, NS, IP == NP - 1>(v, buf, es, tw, t);
+    reg_fft_s<E, DIR, L, IP + 1, NS * R>(v, buf, es, tw, t);
+  }
+}
+
+// reg_fft with the length fixed at compile time when LS > 0 (LS must then
+// equal p.L), the run-time plan otherwise.
+template <int E, int DIR, int LS>
+__device__ __forceinline__ void fft_any(cx (&v)[E], cx *buf, int es, const cx *tw, const FftPlan &p,
+                                        int t, int T) {
+  if constexpr (LS > 0) reg_fft_s<E, DIR, LS>(v, buf, es, tw, t);
+  else reg_fft<E, DIR>(v, buf, es, tw, p, t, T);
+}
+
 }  // namespace rd

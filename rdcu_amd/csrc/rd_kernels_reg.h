@@ -166,13 +166,13 @@ __device__ __forceinline__ cx mult_k(const Grid &g, int blk, const int *nk, cons
 // Forward x pass of rows (plain input or DP45 stage sum) -> S.
 // Block: rpb rows x T threads (T = L/E).  LDS: twiddles L + rpb*padded_len(L)
 // complex.
-template <int E, bool NT = false>
+template <int E, bool NT = false, int LS = 0>
 __device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
                                           int src_kind, const double *src_ptr, const StageCoef &sc,
                                           int rpb) {
   extern __shared__ cx smem[];
-  const int L = g.L, Nx = g.Nx, T = L / E;
+  const int L = LS > 0 ? LS : g.L, Nx = g.Nx, T = L / E;
   const long nrows = (long)g.n * g.rows;
   cx *tw = smem, *buf = smem + L;
   const int tid = threadIdx.x;
@@ -224,7 +224,7 @@ __device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan
   }
   pf_tw.store(tw, px.tw, L, tid, blockDim.x);
   cx *lb = buf + rr * padded_len(L);
-  reg_fft<E, -1>(v, lb, 1, tw, px, t, T);
+  fft_any<E, -1, LS>(v, lb, 1, tw, px, t, T);
 #pragma unroll
   for (int q = 0; q < E; ++q) lb[t + T * q] = v[q];
   __syncthreads();
@@ -248,19 +248,19 @@ __device__ __forceinline__ void rowf_body(long bid, const Grid &g, const FftPlan
   }
 }
 
-template <int E, bool NT>
+template <int E, bool NT, int LS = 0>
 __global__ __launch_bounds__(256) void k_rowf_reg(Grid g, FftPlan px, const cx *__restrict__ twN,
                                                   DevState *st, int check_active, Bufs b,
                                                   int src_kind, const double *src_ptr,
                                                   StageCoef sc, int rpb) {
   if (check_active && !attempt_active(st)) return;
-  rowf_body<E, NT>(blockIdx.x, g, px, twN, st, b, src_kind, src_ptr, sc, rpb);
+  rowf_body<E, NT, LS>(blockIdx.x, g, px, twN, st, b, src_kind, src_ptr, sc, rpb);
 }
 
 // Strided-line pass (modes as k_col).  Block: tc columns x T threads; LDS:
 // twiddles L + tc*padded_len(L) complex (element e of column c at
 // buf[pad8(e)*tc + c]) + tc*L complex holding the forward spectrum (mode 1).
-template <int E, bool NT = false>
+template <int E, bool NT = false, int LS = 0>
 __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan &p,
                                          const ColGeom &cg, const Bufs &b, const FacList &fl,
                                          int mode, int fpw) {
@@ -279,7 +279,7 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     if (m1 > m0 + fpw) m1 = m0 + fpw;
     if (m0 >= m1) return;
   }
-  const int L = cg.L, T = L / E, tc = cg.tc;
+  const int L = LS > 0 ? LS : cg.L, T = L / E, tc = cg.tc;
   const int c0 = ct * tc;
   const int nc = (g.Nxh - c0) < tc ? (g.Nxh - c0) : tc;
   const int tid = threadIdx.x;
@@ -320,15 +320,15 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   if (mode == 1) pf_k.store(k0s, g.kt[0], L, tid, blockDim.x);
   cx *lb = buf + c;
   if (mode != 1) {
-    if (mode == 0) reg_fft<E, -1>(v, lb, tc, tw, p, t, T);
-    else reg_fft<E, +1>(v, lb, tc, tw, p, t, T);
+    if (mode == 0) fft_any<E, -1, LS>(v, lb, tc, tw, p, t, T);
+    else fft_any<E, +1, LS>(v, lb, tc, tw, p, t, T);
     if (act) {
 #pragma unroll
       for (int q = 0; q < E; ++q) st_g<NT>(src + (long)(t + T * q) * cg.es, v[q]);
     }
     return;
   }
-  reg_fft<E, -1>(v, lb, tc, tw, p, t, T);
+  fft_any<E, -1, LS>(v, lb, tc, tw, p, t, T);
   // keep the forward spectrum in LDS (each thread reloads only its own
   // elements, so no barrier is needed) instead of registers across factors
   cx *xs = buf + tc * padded_len(L);
@@ -343,7 +343,7 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
       kv[0] = k0s[t + T * q];
       w[q] = mult_k(g, blk, nk, kv, xs[(t + T * q) * tc + c]);
     }
-    reg_fft<E, +1>(w, lb, tc, tw, p, t, T);
+    fft_any<E, +1, LS>(w, lb, tc, tw, p, t, T);
     if (act) {
       cx *dst = b.F + (long)m * g.Nh + lbase;
 #pragma unroll
@@ -352,14 +352,14 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   }
 }
 
-template <int E, bool NT>
+template <int E, bool NT, int LS = 0>
 __global__ __launch_bounds__(256) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
                                                  int check_active, Bufs b, FacList fl, int mode,
                                                  int fpw) {
   if (check_active && !attempt_active(st)) return;
   const int lblk = xcd_block(blockIdx.x, gridDim.x);
   if (lblk >= cg.nblk) return;
-  col_body<E, NT>(lblk, g, p, cg, b, fl, mode, fpw);
+  col_body<E, NT, LS>(lblk, g, p, cg, b, fl, mode, fpw);
 }
 
 // Inverse x pass for one row position r, all fields: factor lines in groups
@@ -370,11 +370,11 @@ __global__ __launch_bounds__(256) void k_col_reg(Grid g, FftPlan p, ColGeom cg, 
 // Pre-processing of the half-complex -> real transform of factor line m at
 // row position r, elements t + T*q: the split step of the real inverse
 // transform (and in 1-D the multiplier, the factor spectra being S itself).
-template <int E>
+template <int E, int LS = 0>
 __device__ __forceinline__ void c2r_pre(const Grid &g, const FacList &fl, const Bufs &b,
                                         const cx *__restrict__ twN, int m, long r, int t, int T,
                                         cx (&v)[E]) {
-  const int L = g.L;
+  const int L = LS > 0 ? LS : g.L;
   const int blk = fl.fblk[m];
   const cx *row = g.ndim == 1 ? b.S + (long)fl.ffield[m] * g.Nh : b.F + (long)m * g.Nh + r * g.Nxp;
 #pragma unroll
@@ -406,14 +406,14 @@ __host__ __device__ inline long rowi_region(int L, int Nx, int n, int nf, int nw
 // One grid row r, by the nt threads tid = 0..nt-1 of a row group whose LDS
 // region starts at smem.  Every row group of a block runs the same barrier
 // sequence.
-template <int E, class DE>
+template <int E, class DE, int LS = 0>
 __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
                                           const FacList &fl, const DictP &dict, int src_kind,
                                           const double *src_ptr, int dst_kind, int dst_slot,
                                           double *dst_ptr, int err, const ErrCoef &ec,
                                           const StageCoef &nx, int G, cx *smem, int tid, int nt) {
-  const int n = g.n, Nx = g.Nx, L = g.L, T = L / E;
+  const int n = g.n, Nx = g.Nx, L = LS > 0 ? LS : g.L, T = L / E;
   const long N = g.N;
   const int gl = tid / T, t = tid - gl * T;
   const int nf = fl.nf;
@@ -445,7 +445,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   constexpr bool kPre = E <= 8;  // register budget: only for 8 values per thread
   cx v0[kPre ? E : 1];
   if constexpr (kPre) {
-    if (gl < nf) c2r_pre<E>(g, fl, b, twN, gl, r, t, T, v0);
+    if (gl < nf) c2r_pre<E, LS>(g, fl, b, twN, gl, r, t, T, v0);
   }
   pf_tw.store(tw, px.tw, L, tid, nt);
   if constexpr (DE::kProg) pf_p.store(prog, dict.prog, dict.nwords, tid, nt);
@@ -464,12 +464,12 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = v0[kPre ? q : 0];
     } else if (act) {
-      c2r_pre<E>(g, fl, b, twN, m, r, t, T, v);
+      c2r_pre<E, LS>(g, fl, b, twN, m, r, t, T, v);
     } else {
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = cx{0.0, 0.0};
     }
-    reg_fft<E, +1>(v, lb, 1, tw, px, t, T);
+    fft_any<E, +1, LS>(v, lb, 1, tw, px, t, T);
     if (act) {
       double *vr = vals + (long)(n + m) * Nx;
 #pragma unroll
@@ -602,7 +602,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
     const double2 o = *reinterpret_cast<const double2 *>(ys + f * Nx + 2 * (t + T * q));
     v[q] = act ? cx{o.x, o.y} : cx{0.0, 0.0};
   }
-  reg_fft<E, -1>(v, lb, 1, tw, px, t, T);
+  fft_any<E, -1, LS>(v, lb, 1, tw, px, t, T);
 #pragma unroll
   for (int q = 0; q < E; ++q) lb[t + T * q] = v[q];
   __syncthreads();
@@ -629,7 +629,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
 // Block = R rows x G*T threads (R = blockDim / (G*T)), row r = blockIdx*R +
 // sub.  With 24 values per thread the per-row thread range and LDS offset
 // cost registers the kernel does not have, so those kernels take R = 1.
-template <int E, class DE>
+template <int E, class DE, int LS = 0>
 __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
                                           const FacList &fl, const DictP &dict, int src_kind,
@@ -637,15 +637,16 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
                                           double *dst_ptr, int err, const ErrCoef &ec,
                                           const StageCoef &nx, int G) {
   extern __shared__ cx smem[];
+  const int L = LS > 0 ? LS : g.L;
   if constexpr (E <= 8) {
-    const long region = rowi_region(g.L, g.Nx, g.n, fl.nf, dict.nwords, G);
-    const int gt = G * (g.L / E), sub = threadIdx.x / gt;
-    rowi_body<E, DE>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl, dict,
-                     src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
-                     smem + sub * region, threadIdx.x - sub * gt, gt);
+    const long region = rowi_region(L, g.Nx, g.n, fl.nf, dict.nwords, G);
+    const int gt = G * (L / E), sub = threadIdx.x / gt;
+    rowi_body<E, DE, LS>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl, dict,
+                         src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
+                         smem + sub * region, threadIdx.x - sub * gt, gt);
   } else {
-    rowi_body<E, DE>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
-                     dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x);
+    rowi_body<E, DE, LS>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
+                         dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x);
   }
 }
 

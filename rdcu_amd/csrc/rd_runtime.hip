@@ -321,6 +321,30 @@ int reg_elems(int L) {
   return 0;
 }
 
+// Transform lengths with kernels compiled for that length (constant radix
+// plan and index arithmetic); other lengths run the run-time-plan kernels.
+// x lines: the half lengths of 256- and 512-point rows; strided lines: 256
+// and 512 points.
+constexpr int kRowLs[] = {128, 256};
+constexpr int kColLs[] = {256, 512};
+
+template <int E, bool NT, int LS>
+void go_rowf(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, int check, int src_kind,
+             const double *src_ptr, const rd::StageCoef &sc, int rpb) {
+  hipLaunchKernelGGL((rd::k_rowf_reg<E, NT, LS>), g, b, lds, c->stream, c->grid, c->px, c->twN,
+                     c->st, check, c->bufs, src_kind, src_ptr, sc, rpb);
+}
+
+template <int E, bool NT>
+void go_rowf_l(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, int check, int src_kind,
+               const double *src_ptr, const rd::StageCoef &sc, int rpb) {
+  if constexpr (E == 8) {
+    if (c->L == kRowLs[0]) return go_rowf<E, NT, kRowLs[0]>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
+    if (c->L == kRowLs[1]) return go_rowf<E, NT, kRowLs[1]>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
+  }
+  go_rowf<E, NT, 0>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
+}
+
 template <int E>
 void launch_rowf(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, rd::StageCoef sc,
                  int rpb) {
@@ -329,11 +353,26 @@ void launch_rowf(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, rd
   dim3 g((unsigned)((nrows + rpb - 1) / rpb));
   size_t lds = (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx);
   if (c->grid.nt_store)
-    hipLaunchKernelGGL((rd::k_rowf_reg<E, true>), g, dim3(rpb * T), lds, c->stream, c->grid, c->px,
-                       c->twN, c->st, check, c->bufs, src_kind, src_ptr, sc, rpb);
+    go_rowf_l<E, true>(c, g, dim3(rpb * T), lds, check, src_kind, src_ptr, sc, rpb);
   else
-    hipLaunchKernelGGL((rd::k_rowf_reg<E, false>), g, dim3(rpb * T), lds, c->stream, c->grid, c->px,
-                       c->twN, c->st, check, c->bufs, src_kind, src_ptr, sc, rpb);
+    go_rowf_l<E, false>(c, g, dim3(rpb * T), lds, check, src_kind, src_ptr, sc, rpb);
+}
+
+template <int E, bool NT, int LS>
+void go_col(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, const rd::FftPlan &p, const rd::ColGeom &cg,
+            int check, const rd::FacList &fl, int mode, int fpw) {
+  hipLaunchKernelGGL((rd::k_col_reg<E, NT, LS>), g, b, lds, c->stream, c->grid, p, cg, c->st,
+                     check, c->bufs, fl, mode, fpw);
+}
+
+template <int E, bool NT>
+void go_col_l(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, const rd::FftPlan &p, const rd::ColGeom &cg,
+              int check, const rd::FacList &fl, int mode, int fpw) {
+  if constexpr (E == 8 || E == 16) {
+    if (cg.L == kColLs[0]) return go_col<E, NT, kColLs[0]>(c, g, b, lds, p, cg, check, fl, mode, fpw);
+    if (cg.L == kColLs[1]) return go_col<E, NT, kColLs[1]>(c, g, b, lds, p, cg, check, fl, mode, fpw);
+  }
+  go_col<E, NT, 0>(c, g, b, lds, p, cg, check, fl, mode, fpw);
 }
 
 template <int E>
@@ -349,11 +388,9 @@ void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups
                    sizeof(cx) +
                (mode == 1 ? (size_t)cg.L * sizeof(double) : 0);
   if (c->grid.nt_store)
-    hipLaunchKernelGGL((rd::k_col_reg<E, true>), g, dim3(cg.tc * T), lds, c->stream, c->grid, p, cg,
-                       c->st, check, c->bufs, fl, mode, fpw);
+    go_col_l<E, true>(c, g, dim3(cg.tc * T), lds, p, cg, check, fl, mode, fpw);
   else
-    hipLaunchKernelGGL((rd::k_col_reg<E, false>), g, dim3(cg.tc * T), lds, c->stream, c->grid, p, cg,
-                       c->st, check, c->bufs, fl, mode, fpw);
+    go_col_l<E, false>(c, g, dim3(cg.tc * T), lds, p, cg, check, fl, mode, fpw);
 }
 
 size_t rowi_lds(const rdcu_ctx *c, int G, int R) {
@@ -639,8 +676,9 @@ void build_jit(rdcu_ctx *c) {
          "int dst_slot, double *dst_ptr, int err, rd::ErrCoef ec, rd::StageCoef nx, int G) {\n";
   const std::string E = std::to_string(c->Ex);
   src += "  if (check_active && !rd::attempt_active(st)) return;\n";
-  src += "  rd::rowi_rows<" + E +
-         ", rd::JitDict>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot, "
+  // the x-line length is a compile-time constant of the generated kernel
+  src += "  rd::rowi_rows<" + E + ", rd::JitDict, " + std::to_string(c->L) +
+         ">(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot, "
          "dst_ptr, err, ec, nx, G);\n}\n";
   std::vector<char> code;
   if (!jit_compile(src, jit_arch(c->dev), code, c->jit_log)) return;
@@ -810,7 +848,19 @@ int allow_big_lds() {
       (const void *)rd::k_col_reg<16, false>,  (const void *)rd::k_col_reg<16, true>,
       (const void *)rd::k_col_reg<24, false>,  (const void *)rd::k_col_reg<24, true>,
       (const void *)rd::k_rowi_reg<8>,         (const void *)rd::k_rowi_reg<24>,
-      (const void *)rd::k_rowi_vjp<8>,         (const void *)rd::k_rowi_vjp<24>};
+      (const void *)rd::k_rowi_vjp<8>,         (const void *)rd::k_rowi_vjp<24>,
+      (const void *)rd::k_rowf_reg<8, false, kRowLs[0]>,
+      (const void *)rd::k_rowf_reg<8, true, kRowLs[0]>,
+      (const void *)rd::k_rowf_reg<8, false, kRowLs[1]>,
+      (const void *)rd::k_rowf_reg<8, true, kRowLs[1]>,
+      (const void *)rd::k_col_reg<8, false, kColLs[0]>,
+      (const void *)rd::k_col_reg<8, true, kColLs[0]>,
+      (const void *)rd::k_col_reg<8, false, kColLs[1]>,
+      (const void *)rd::k_col_reg<8, true, kColLs[1]>,
+      (const void *)rd::k_col_reg<16, false, kColLs[0]>,
+      (const void *)rd::k_col_reg<16, true, kColLs[0]>,
+      (const void *)rd::k_col_reg<16, false, kColLs[1]>,
+      (const void *)rd::k_col_reg<16, true, kColLs[1]>};
   for (const void *f : fs)
     HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return RDCU_OK;
