@@ -370,18 +370,32 @@ __global__ __launch_bounds__(256) void k_col_reg(Grid g, FftPlan p, ColGeom cg, 
 // Pre-processing of the half-complex -> real transform of factor line m at
 // row position r, elements t + T*q: the split step of the real inverse
 // transform (and in 1-D the multiplier, the factor spectra being S itself).
-template <int E, int LS = 0>
+// Problem sizes of a kernel compiled for one problem (the generated
+// x-inverse kernels): nonzero members (nf: >= 0) are compile-time constants
+// equal to the launch's Grid and FacList values; RunGeom reads those instead.
+struct RunGeom {
+  static constexpr int n = 0, ndim = 0, Nx = 0, Nxp = 0, nf = -1;
+  static constexpr long N = 0, rows = 0;
+};
+
+template <int E, int LS = 0, class GK = RunGeom>
 __device__ __forceinline__ void c2r_pre(const Grid &g, const FacList &fl, const Bufs &b,
                                         const cx *__restrict__ twN, int m, long r, int t, int T,
                                         cx (&v)[E]) {
   const int L = LS > 0 ? LS : g.L;
-  const int blk = fl.fblk[m];
-  const cx *row = g.ndim == 1 ? b.S + (long)fl.ffield[m] * g.Nh : b.F + (long)m * g.Nh + r * g.Nxp;
+  const long rows = GK::rows ? GK::rows : g.rows;
+  const int Nxp = GK::Nxp ? GK::Nxp : g.Nxp;
+  const long Nh = rows * Nxp;
+  // the factor tables are read only in 1-D (a per-thread load whose result
+  // the row address would otherwise wait for)
+  const bool d1 = (GK::ndim ? GK::ndim : g.ndim) == 1;
+  const int blk = d1 ? fl.fblk[m] : 0;
+  const cx *row = d1 ? b.S + (long)fl.ffield[m] * Nh : b.F + (long)m * Nh + r * Nxp;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int k = t + T * q;
     cx Av = row[k], Bv = row[L - k];
-    if (g.ndim == 1) {
+    if (d1) {
       int nk0 = k, nk1 = L - k;
       Av = apply_mult(g, blk, &nk0, Av);
       Bv = apply_mult(g, blk, &nk1, Bv);
@@ -406,17 +420,19 @@ __host__ __device__ inline long rowi_region(int L, int Nx, int n, int nf, int nw
 // One grid row r, by the nt threads tid = 0..nt-1 of a row group whose LDS
 // region starts at smem.  Every row group of a block runs the same barrier
 // sequence.
-template <int E, class DE, int LS = 0>
+template <int E, class DE, int LS = 0, class GK = RunGeom>
 __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
                                           const FacList &fl, const DictP &dict, int src_kind,
                                           const double *src_ptr, int dst_kind, int dst_slot,
                                           double *dst_ptr, int err, const ErrCoef &ec,
                                           const StageCoef &nx, int G, cx *smem, int tid, int nt) {
-  const int n = g.n, Nx = g.Nx, L = LS > 0 ? LS : g.L, T = L / E;
-  const long N = g.N;
+  const int n = GK::n ? GK::n : g.n, Nx = GK::Nx ? GK::Nx : g.Nx;
+  const int L = LS > 0 ? LS : g.L, T = L / E;
+  const long N = GK::N ? GK::N : g.N, rows = GK::rows ? GK::rows : g.rows;
+  const int Nxp = GK::Nxp ? GK::Nxp : g.Nxp;
   const int gl = tid / T, t = tid - gl * T;
-  const int nf = fl.nf;
+  const int nf = GK::nf >= 0 ? GK::nf : fl.nf;
   const int PL = padded_len(L);
   cx *tw = smem, *buf = smem + L;
   double *vals = reinterpret_cast<double *>(buf + G * PL);
@@ -429,23 +445,25 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   const double *src =
       src_kind == SRC_Y ? b.y[st->iy] : (src_kind == SRC_YT ? b.y[st->iyt] : src_ptr);
   // Prologue: every global load is issued before the first LDS store or
-  // use, so they share one memory round trip (identity rows, twiddles,
-  // program, and for E = 8 the first factor group's pre-processed spectrum).
+  // use, so they share one memory round trip: for E = 8 the first factor
+  // group's pre-processed spectrum (its addresses need no other load;
+  // computed by every line group, without a divergent branch, and kept by
+  // those with a factor), the twiddles, the program and the identity rows.
+  constexpr bool kPre = E <= 8;  // register budget: only for 8 values per thread
+  cx v0[kPre ? E : 1];
+  if constexpr (kPre) {
+    if (nf > 0) c2r_pre<E, LS, GK>(g, fl, b, twN, gl < nf ? gl : 0, r, t, T, v0);
+  }
+  Prefetch<4, cx> pf_tw;
+  pf_tw.load(px.tw, L, tid, nt);
+  Prefetch<4, int> pf_p;
+  if constexpr (DE::kProg) pf_p.load(dict.prog, dict.nwords, tid, nt);
   constexpr int IDU = 8;
   double idr[IDU];
 #pragma unroll
   for (int u = 0; u < IDU; ++u) {
     const int w = tid + u * nt;
     idr[u] = w < n * Nx ? src[(w / Nx) * N + r * Nx + (w % Nx)] : 0.0;
-  }
-  Prefetch<4, cx> pf_tw;
-  pf_tw.load(px.tw, L, tid, nt);
-  Prefetch<4, int> pf_p;
-  if constexpr (DE::kProg) pf_p.load(dict.prog, dict.nwords, tid, nt);
-  constexpr bool kPre = E <= 8;  // register budget: only for 8 values per thread
-  cx v0[kPre ? E : 1];
-  if constexpr (kPre) {
-    if (gl < nf) c2r_pre<E, LS>(g, fl, b, twN, gl, r, t, T, v0);
   }
   pf_tw.store(tw, px.tw, L, tid, nt);
   if constexpr (DE::kProg) pf_p.store(prog, dict.prog, dict.nwords, tid, nt);
@@ -464,7 +482,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = v0[kPre ? q : 0];
     } else if (act) {
-      c2r_pre<E, LS>(g, fl, b, twN, m, r, t, T, v);
+      c2r_pre<E, LS, GK>(g, fl, b, twN, m, r, t, T, v);
     } else {
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = cx{0.0, 0.0};
@@ -547,7 +565,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
       }
       __syncthreads();
     }
-    if (tid < n) b.part[(long)tid * g.rows + r] = sq[tid * Nx];
+    if (tid < n) b.part[(long)tid * rows + r] = sq[tid * Nx];
   }
   if (nx.ns == 0) return;
 
@@ -570,21 +588,27 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
       const long off = f * N + r * Nx + j2;
       const double2 yv2 = *reinterpret_cast<const double2 *>(y + off);
       const double2 a1 = *reinterpret_cast<const double2 *>(ks[0] + off);
+      // loads issued unconditionally (absent terms read k1 again and are
+      // discarded), so they share one round trip
+      double2 kk[5];
+#pragma unroll
+      for (int s = 1; s < 6; ++s) {
+        const bool use = s < nx.ns - 1 && nx.a[s] != 0.0;
+        kk[s - 1] = *reinterpret_cast<const double2 *>((use ? ks[s] : ks[0]) + off);
+      }
       double ar = nx.a[0] * a1.x, ai = nx.a[0] * a1.y;
 #pragma unroll
       for (int s = 1; s < 6; ++s) {  // unrolled: ks[] stays in registers
-        if (s >= nx.ns || nx.a[s] == 0.0) continue;
-        double kx_, ky_;
+        const double as = s < nx.ns ? nx.a[s] : 0.0;
+        double kx_ = kk[s - 1].x, ky_ = kk[s - 1].y;
         if (s == nx.ns - 1) {  // the stage just computed
           kx_ = fv[f * Nx + j2];
           ky_ = fv[f * Nx + j2 + 1];
-        } else {
-          const double2 kk = *reinterpret_cast<const double2 *>(ks[s] + off);
-          kx_ = kk.x;
-          ky_ = kk.y;
         }
-        ar = __builtin_fma(nx.a[s], kx_, ar);
-        ai = __builtin_fma(nx.a[s], ky_, ai);
+        const double nr = __builtin_fma(as, kx_, ar);
+        const double ni = __builtin_fma(as, ky_, ai);
+        ar = as != 0.0 ? nr : ar;
+        ai = as != 0.0 ? ni : ai;
       }
       double2 o;
       o.x = __builtin_fma(hs, ar, yv2.x);
@@ -607,7 +631,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   for (int q = 0; q < E; ++q) lb[t + T * q] = v[q];
   __syncthreads();
   if (!act) return;
-  cx *S = b.S + ((long)f * g.rows + r) * g.Nxp;
+  cx *S = b.S + ((long)f * rows + r) * Nxp;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int k = t + T * q;
@@ -629,7 +653,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
 // Block = R rows x G*T threads (R = blockDim / (G*T)), row r = blockIdx*R +
 // sub.  With 24 values per thread the per-row thread range and LDS offset
 // cost registers the kernel does not have, so those kernels take R = 1.
-template <int E, class DE, int LS = 0>
+template <int E, class DE, int LS = 0, class GK = RunGeom>
 __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
                                           const FacList &fl, const DictP &dict, int src_kind,
@@ -639,14 +663,16 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
   extern __shared__ cx smem[];
   const int L = LS > 0 ? LS : g.L;
   if constexpr (E <= 8) {
-    const long region = rowi_region(L, g.Nx, g.n, fl.nf, dict.nwords, G);
+    const int Nx = GK::Nx ? GK::Nx : g.Nx, n = GK::n ? GK::n : g.n;
+    const int nf = GK::nf >= 0 ? GK::nf : fl.nf;
+    const long region = rowi_region(L, Nx, n, nf, dict.nwords, G);
     const int gt = G * (L / E), sub = threadIdx.x / gt;
-    rowi_body<E, DE, LS>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl, dict,
-                         src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
-                         smem + sub * region, threadIdx.x - sub * gt, gt);
+    rowi_body<E, DE, LS, GK>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl,
+                             dict, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
+                             smem + sub * region, threadIdx.x - sub * gt, gt);
   } else {
-    rowi_body<E, DE, LS>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
-                         dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x);
+    rowi_body<E, DE, LS, GK>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
+                             dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x);
   }
 }
 

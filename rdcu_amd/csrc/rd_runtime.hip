@@ -676,9 +676,18 @@ void build_jit(rdcu_ctx *c) {
          "int dst_slot, double *dst_ptr, int err, rd::ErrCoef ec, rd::StageCoef nx, int G) {\n";
   const std::string E = std::to_string(c->Ex);
   src += "  if (check_active && !rd::attempt_active(st)) return;\n";
-  // the x-line length is a compile-time constant of the generated kernel
+  // The grid sizes and the factor count are constants of the generated
+  // kernel (index arithmetic folds, and the 1-D-only table reads disappear
+  // in 2-D and 3-D); the launch passes the same values in g and fl.
+  const rd::Grid &gd = c->grid;
+  char buf[512];
+  snprintf(buf, sizeof buf,
+           "namespace rd {\nstruct JitGeom {\n  static constexpr int n = %d, ndim = %d, Nx = %d, "
+           "Nxp = %d, nf = %d;\n  static constexpr long N = %ldL, rows = %ldL;\n};\n}\n",
+           gd.n, gd.ndim, gd.Nx, gd.Nxp, c->fl_used.nf, gd.N, gd.rows);
+  src = buf + src;
   src += "  rd::rowi_rows<" + E + ", rd::JitDict, " + std::to_string(c->L) +
-         ">(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot, "
+         ", rd::JitGeom>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot, "
          "dst_ptr, err, ec, nx, G);\n}\n";
   std::vector<char> code;
   if (!jit_compile(src, jit_arch(c->dev), code, c->jit_log)) return;
