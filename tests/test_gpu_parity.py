@@ -230,3 +230,27 @@ def test_sixteen_values_per_thread_columns_bitwise(hlib, olib, monkeypatch, Ns, 
             to, ho, ao = o.step(to, ho)
             assert (tg, hg, ag) == (to, ho, ao)
         assert np.array_equal(g.state(), o.state())
+
+
+@pytest.mark.parametrize("Ns,Ls,shapes", [([512, 512], [200.0, 200.0], "8,4,2,4,2,8"),
+                                          ([512, 512], [200.0, 200.0], "8,4,1,4,2,8"),
+                                          ([256, 256], [100.0, 100.0], "4,4,2,4,2,16"),
+                                          ([256, 16, 32], [60.0, 20.0, 30.0], "4,4,2,4,1,8")],
+                         ids=["2d-512-fpw2", "2d-512-fpw1", "2d-256-e16", "3d-256z"])
+def test_compiled_length_kernels_bitwise(hlib, olib, monkeypatch, Ns, Ls, shapes):
+    """Kernels compiled for one transform length (128/256-point x lines,
+    256/512-point strided lines), with two factors per block (fpw = 2) or one
+    (fpw = 1), 8 or 16 values per thread: the oracle's bits."""
+    prob = make_problem(Ns, Ls, "cgle")
+    y0 = glibc_rand_ic(prob.n, prob.N, seed=7, A=1.0)
+    monkeypatch.setenv("RDCU_SHAPES", shapes)
+    print(f"case compiled lengths {Ns} {shapes}", flush=True)
+    with Integrator(hlib, prob, y0) as g, Integrator(olib, prob, y0) as o:
+        assert np.array_equal(g.rhs(y0), o.rhs(y0))
+        tg = to = 0.0
+        hg = ho = 0.01
+        for i in range(3):
+            tg, hg, ag = g.step(tg, hg)
+            to, ho, ao = o.step(to, ho)
+            assert (tg, hg, ag) == (to, ho, ao)
+        assert np.array_equal(g.state(), o.state())
