@@ -161,7 +161,7 @@ def sweep_leg(lib, count, t_end, every):
     return out
 
 
-def adjoint_leg(lib):
+def adjoint_leg(lib, nbatch=0, concurrency=4):
     """Gradient of an output-matching loss by the discrete adjoint.
 
     2-D CGLE 256x256, L=100, IC amplitude 1 seed 1, rtol=atol=1e-10, outputs
@@ -170,7 +170,14 @@ def adjoint_leg(lib):
     coefficients on the step schedule of their adaptive run.  Reports the
     time of one loss-only evaluation and of loss + gradient (all 18
     coefficients and the initial state), and central differences (h=1e-6) on
-    three coefficients against the adjoint."""
+    three coefficients against the adjoint.
+
+    With nbatch > 0, also the gradients of nbatch instances at once
+    (rdcu_loss_grad_batch, `concurrency` instances on the GPU together): a
+    population of coefficient points (nominal with 1% relative Gaussian
+    noise, seed 1) fitted to the same target on the nominal schedule, timed
+    against nbatch times one instance's loss + gradient; the first instance's
+    batched result is checked bit for bit against its own single call."""
     import numpy as np
     from rdcu_amd import Integrator, Problem, cgle_terms, glibc_rand_ic
     prob = Problem(2, [256, 256], [100.0, 100.0], cgle_terms(2, 2.0, 0.8))
@@ -217,6 +224,31 @@ def adjoint_leg(lib):
            "schedule_s": a1 - a0, "loss_s": a3 - a2, "loss_grad_s": a4 - a3,
            "grad_over_loss": (a4 - a3) / (a3 - a2), "loss": L2,
            "fd_max_rel_err_3coef": max(errs), "grad_norm_coef": float(np.linalg.norm(gc))}
+    if nbatch > 0:
+        from rdcu_amd import loss_grad_batch
+        rng1 = np.random.default_rng(1)
+        b0 = time.perf_counter()
+        its = []
+        for i in range(nbatch):
+            it_i = Integrator(lib, prob, y0, atol=1e-10, rtol=1e-10)
+            it_i.set_coef(coef * (1 + 0.01 * rng1.standard_normal(coef.size)))
+            its.append(it_i)
+        # companion problems are built by the first gradient call
+        loss_grad_batch(its, [y0] * nbatch, [0.0] * nbatch, [hs[:1]] * nbatch,
+                        [hs[:1]] * nbatch, [y0] * nbatch, concurrency=concurrency)
+        b1 = time.perf_counter()
+        got = loss_grad_batch(its, [y0] * nbatch, [0.0] * nbatch, [hs] * nbatch,
+                              [t_out] * nbatch, [target] * nbatch, concurrency=concurrency)
+        b2 = time.perf_counter()
+        L1, gc1, gy1 = its[0].loss_grad(y0, 0.0, hs, t_out, target)
+        same = bool(got[0][0] == L1 and np.array_equal(got[0][1], gc1)
+                    and np.array_equal(got[0][2], gy1))
+        for it_i in its:
+            it_i.close()
+        out["batch"] = {"instances": nbatch, "concurrency": concurrency, "setup_s": b1 - b0,
+                        "batch_s": b2 - b1, "per_instance_s": (b2 - b1) / nbatch,
+                        "speedup_vs_single": nbatch * (a4 - a3) / (b2 - b1),
+                        "first_identical_to_single": same}
     log("adjoint:", json.dumps(out))
     return out
 
@@ -249,6 +281,10 @@ def main():
                     help="serial baseline on every k-th sweep instance")
     ap.add_argument("--adjoint", type=int, default=1,
                     help="1: time the discrete adjoint on 2-D CGLE 256x256, t=10 (0 to skip)")
+    ap.add_argument("--adjoint-batch", type=int, default=64,
+                    help="instances of the batched adjoint (0 to skip)")
+    ap.add_argument("--adjoint-concurrency", type=int, default=4,
+                    help="instances of the batched adjoint on the GPU together")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each instance lives "
@@ -361,7 +397,7 @@ def main():
     if args.sweep > 0 and nd == 2:
         result["batched"] = sweep_leg(lib, args.sweep, args.sweep_t, args.serial_every)
     if args.adjoint and nd == 2:
-        result["adjoint"] = adjoint_leg(lib)
+        result["adjoint"] = adjoint_leg(lib, args.adjoint_batch, args.adjoint_concurrency)
     print(json.dumps(result), flush=True)
 
 

@@ -185,6 +185,29 @@ int rdcu_loss_grad(rdcu_ctx *ctx, const double *y0, double t0, long long nsteps,
                    const double *hs, int nout, const double *t_out, const double *target,
                    double *loss, double *grad_coef, double *grad_y0);
 
+/* One instance of a gradient batch: the arguments of rdcu_loss_grad, and its
+ * status (an RDCU_* code) on return. */
+typedef struct rdcu_grad_job {
+  rdcu_ctx *ctx;
+  const double *y0;
+  double t0;
+  long long nsteps;
+  const double *hs;
+  int nout;
+  const double *t_out;
+  const double *target;
+  double *loss;
+  double *grad_coef;
+  double *grad_y0;
+  int status;
+} rdcu_grad_job;
+
+/* rdcu_loss_grad for jobs[0 .. count), each on its own handle (all distinct),
+ * up to `concurrency` of them at a time (<= 0: a default).  Every job gives
+ * exactly what rdcu_loss_grad gives for it alone.  Returns RDCU_OK, or the
+ * first failing job's status (its message is then the last error). */
+int rdcu_loss_grad_batch(rdcu_grad_job *jobs, int count, int concurrency);
+
 /* Counters: out[0] attempts, out[1] accepted, out[2] RHS evaluations,
  * out[3] host synchronisations (chunks). */
 int rdcu_get_stats(rdcu_ctx *ctx, long long out[4]);

@@ -1205,6 +1205,31 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
   return RDCU_OK;
 }
 
+// Batched gradients: the jobs one after the other (each parallel inside).
+int rdcu_loss_grad_batch(rdcu_grad_job *jobs, int count, int concurrency) {
+  (void)concurrency;
+  if (count < 0 || (count > 0 && !jobs)) return fail(RDCU_EINVAL, "bad argument");
+  for (int i = 0; i < count; ++i) {
+    jobs[i].status = RDCU_OK;
+    if (!jobs[i].ctx) return fail(RDCU_EINVAL, "every job needs its own handle");
+    for (int k = 0; k < i; ++k)
+      if (jobs[k].ctx == jobs[i].ctx) return fail(RDCU_EINVAL, "every job needs its own handle");
+  }
+  int first = -1;
+  std::string msg;
+  for (int i = 0; i < count; ++i) {
+    rdcu_grad_job &j = jobs[i];
+    j.status = rdcu_loss_grad(j.ctx, j.y0, j.t0, j.nsteps, j.hs, j.nout, j.t_out, j.target,
+                              j.loss, j.grad_coef, j.grad_y0);
+    if (j.status && first < 0) {
+      first = i;
+      msg = g_err;
+    }
+  }
+  if (first >= 0) return fail(jobs[first].status, msg);
+  return RDCU_OK;
+}
+
 int rdcu_time_rhs(rdcu_ctx *c, int reps, double *ms, int max, int *count, char *names,
                   int names_len) {
   if (!c || !ms || max < 1 || reps < 1) return fail(RDCU_EINVAL, "bad argument");

@@ -33,6 +33,7 @@ SYMBOLS = (
     "rdcu_last_error", "rdcu_backend",
     "rdcu_batch_create", "rdcu_batch_destroy", "rdcu_batch_step", "rdcu_batch_run",
     "rdcu_batch_member", "rdcu_set_coef", "rdcu_schedule", "rdcu_loss_grad",
+    "rdcu_loss_grad_batch",
 )
 
 
@@ -52,6 +53,16 @@ class _Problem(ctypes.Structure):
 
 
 _DP = ctypes.POINTER(ctypes.c_double)
+
+
+class _GradJob(ctypes.Structure):
+    """rdcu_grad_job"""
+    _fields_ = [
+        ("ctx", ctypes.c_void_p), ("y0", _DP), ("t0", ctypes.c_double),
+        ("nsteps", ctypes.c_longlong), ("hs", _DP), ("nout", ctypes.c_int), ("t_out", _DP),
+        ("target", _DP), ("loss", _DP), ("grad_coef", _DP), ("grad_y0", _DP),
+        ("status", ctypes.c_int),
+    ]
 STEP_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                            ctypes.c_int, ctypes.c_int, _DP, ctypes.c_void_p)
 
@@ -98,6 +109,7 @@ def load(path: str) -> ctypes.CDLL:
                                   ctypes.POINTER(ctypes.c_longlong)]
     lib.rdcu_loss_grad.argtypes = [vp, _DP, ctypes.c_double, ctypes.c_longlong, _DP, ctypes.c_int,
                                    _DP, _DP, _DP, _DP, _DP]
+    lib.rdcu_loss_grad_batch.argtypes = [ctypes.POINTER(_GradJob), ctypes.c_int, ctypes.c_int]
     lib.rdcu_last_error.restype = ctypes.c_char_p
     lib.rdcu_backend.restype = ctypes.c_char_p
     for s in SYMBOLS:
@@ -429,6 +441,45 @@ class Batch:
         self._check(self.lib.rdcu_batch_run(self.h, _ptr(tt), _ptr(hh), hmin, hmax, _ptr(t1),
                                             int(max_attempts), na))
         return tt, hh, np.array(na[:], dtype=np.int64)
+
+
+def loss_grad_batch(integrators: Sequence[Integrator], y0s, t0s, hss, t_outs, targets,
+                    grad=True, concurrency=4):
+    """Integrator.loss_grad for many instances at once (rdcu_loss_grad_batch:
+    up to `concurrency` instances on the GPU together, each on its own
+    handle).  Returns [(loss, dloss/dcoef, dloss/dy0)] (or [loss] without
+    grad), each exactly what loss_grad gives for that instance alone."""
+    its = list(integrators)
+    if len({id(it) for it in its}) != len(its):
+        raise ValueError("every instance needs its own Integrator")
+    lib = its[0].lib if its else None
+    keep, out = [], []
+    jobs = (_GradJob * max(1, len(its)))()
+    for i, it in enumerate(its):
+        if it.lib is not lib:
+            raise ValueError("all instances must use one library")
+        y0 = np.ascontiguousarray(y0s[i], dtype=np.float64).reshape(-1)
+        hs = np.ascontiguousarray(hss[i], dtype=np.float64).reshape(-1)
+        to = np.ascontiguousarray(t_outs[i], dtype=np.float64).reshape(-1)
+        tg = np.ascontiguousarray(targets[i], dtype=np.float64).reshape(-1)
+        if y0.size != it.nN or tg.size != to.size * it.nN:
+            raise ValueError("y0 needs n*N values and target nout*n*N")
+        loss = np.zeros(1)
+        gc = np.empty(max(1, len(it.prob.terms))) if grad else None
+        gy = np.empty(it.nN) if grad else None
+        keep += [y0, hs, to, tg, loss, gc, gy]
+        jobs[i] = _GradJob(it.h, _ptr(y0), float(t0s[i]), hs.size, _ptr(hs), to.size, _ptr(to),
+                           _ptr(tg), _ptr(loss), _ptr(gc) if grad else None,
+                           _ptr(gy) if grad else None, 0)
+        out.append((loss, gc, gy, len(it.prob.terms)))
+    if not its:
+        return []
+    code = lib.rdcu_loss_grad_batch(jobs, len(its), int(concurrency))
+    if code:
+        raise RdcuError(code, lib.rdcu_last_error().decode())
+    if not grad:
+        return [o[0][0] for o in out]
+    return [(o[0][0], o[1][:o[3]], o[2]) for o in out]
 
 
 def glibc_rand_ic(n: int, N: int, seed: int = 1, A: float = 1e-3, A0: float = 0.0) -> np.ndarray:

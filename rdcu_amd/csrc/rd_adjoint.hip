@@ -12,18 +12,28 @@
 
 namespace {
 
-// RAII list of device allocations of one call.
+// RAII list of device allocations of one call, ordered on the call's stream
+// (stream-ordered allocation: no device-wide synchronisation, so calls on
+// other streams keep running; plain hipMalloc where pools are unsupported).
 struct DevScratch {
-  std::vector<void *> p;
+  hipStream_t s;
+  std::vector<std::pair<void *, bool>> p;  // (pointer, stream-ordered)
+  explicit DevScratch(hipStream_t st) : s(st) {}
   ~DevScratch() {
-    for (void *q : p) (void)hipFree(q);
+    for (auto &q : p) (void)(q.second ? hipFreeAsync(q.first, s) : hipFree(q.first));
   }
   template <class T>
   int get(T **out, size_t count) {
     void *q = nullptr;
-    hipError_t e = hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T));
-    if (e != hipSuccess) return fail(RDCU_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    p.push_back(q);
+    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    bool async = hipMallocAsync(&q, bytes, s) == hipSuccess;
+    if (!async) {
+      (void)hipGetLastError();
+      hipError_t e = hipMalloc(&q, bytes);
+      if (e != hipSuccess)
+        return fail(RDCU_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    p.push_back({q, async});
     *out = static_cast<T *>(q);
     return RDCU_OK;
   }
@@ -193,7 +203,7 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
   const long N = c->N, nN = (long)c->n * N;
   const double cnt = (double)nout * (double)nN;
   const unsigned nbl = blocks_of(nN);
-  DevScratch ds;
+  DevScratch ds(c->stream);
   double *d_hs, *d_tg, *d_part, *ych = nullptr, *gbar = nullptr;
   if ((st = ds.get(&d_hs, ns)) || (st = ds.get(&d_tg, (size_t)nout * nN)) ||
       (st = ds.get(&d_part, (size_t)nout * nbl)))
@@ -327,6 +337,51 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
       for (long r = 0; r < c->rows; ++r) sacc += hp[(size_t)t * c->rows + r];
       grad_coef[t] = sacc;
     }
+  return RDCU_OK;
+}
+
+int rdcu_loss_grad_batch(rdcu_grad_job *jobs, int count, int concurrency) {
+  if (count < 0 || (count > 0 && !jobs)) return fail(RDCU_EINVAL, "bad argument");
+  std::set<rdcu_ctx *> seen;
+  for (int i = 0; i < count; ++i) {
+    jobs[i].status = RDCU_OK;
+    if (!jobs[i].ctx || !seen.insert(jobs[i].ctx).second)
+      return fail(RDCU_EINVAL, "every job needs its own handle");
+  }
+  // companion problems (they compile and tune) are built here, one at a time
+  for (int i = 0; i < count; ++i) {
+    rdcu_grad_job &j = jobs[i];
+    if (!(j.grad_coef || j.grad_y0) || !j.ctx->reg || j.ctx->member) continue;
+    HIPCHECK(hipSetDevice(j.ctx->dev));
+    if (int st = adj_companion(j.ctx)) return j.status = st;
+  }
+  // Each worker thread runs whole jobs, each on its job's own stream, so up
+  // to `concurrency` instances' kernels share the GPU.
+  const int W = std::max(1, std::min(count, concurrency > 0 ? concurrency : 4));
+  std::atomic<int> next{0};
+  std::mutex mu;
+  int first = -1;
+  std::string first_msg;
+  auto work = [&]() {
+    for (;;) {
+      const int i = next.fetch_add(1);
+      if (i >= count) break;
+      rdcu_grad_job &j = jobs[i];
+      j.status = rdcu_loss_grad(j.ctx, j.y0, j.t0, j.nsteps, j.hs, j.nout, j.t_out, j.target,
+                                j.loss, j.grad_coef, j.grad_y0);
+      if (j.status) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (first < 0 || i < first) {
+          first = i;
+          first_msg = rdcu_last_error();
+        }
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int w = 0; w < W; ++w) th.emplace_back(work);
+  for (auto &t : th) t.join();
+  if (first >= 0) return fail(jobs[first].status, first_msg);
   return RDCU_OK;
 }
 

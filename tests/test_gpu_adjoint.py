@@ -109,3 +109,41 @@ def test_adjoint_config_vs_central_differences(hlib, olib):
         Lt, dL = loss_tl(olib, o, y0, 0.0, hs[:k], t_out[:1], target[:y0.size], dc, dy)
     assert Lt == pytest.approx(L1, rel=1e-12)
     assert float(gc1 @ dc + gy1 @ dy) == pytest.approx(dL, rel=1e-6)
+
+
+def test_batched_gradients_equal_single_calls(hlib, olib):
+    """rdcu_loss_grad_batch: 6 instances (own coefficients, states, schedules
+    and targets) three at a time on the GPU give bit for bit what one
+    rdcu_loss_grad call gives each; the first also agrees with the oracle."""
+    from rdcu_amd import loss_grad_batch
+    prob = Problem(2, [32, 48], [30.0, 40.0], cgle_terms(2, 2.0, 0.8))
+    rng = np.random.default_rng(21)
+    base = np.array([t.coef for t in prob.terms])
+    its, y0s, hss, touts, tgs, coefs = [], [], [], [], [], []
+    print("case batched gradients", flush=True)
+    for i in range(6):
+        y0 = glibc_rand_ic(2, prob.N, seed=40 + i, A=1.0)
+        coef = base * (1 + 0.05 * rng.standard_normal(base.size))
+        it = Integrator(hlib, prob, y0, atol=1e-10, rtol=1e-10)
+        it.set_coef(coef)
+        hs = it.schedule(0.0, 0.01, 0.2)
+        t_out = np.array([0.1, 0.2])
+        its.append(it)
+        y0s.append(y0)
+        hss.append(hs)
+        touts.append(t_out)
+        coefs.append(coef)
+        tgs.append(y0[None, :] * 0.9 + 0.01 * rng.standard_normal((2, y0.size)))
+    got = loss_grad_batch(its, y0s, [0.0] * 6, hss, touts, tgs, concurrency=3)
+    for i, it in enumerate(its):
+        L, gc, gy = it.loss_grad(y0s[i], 0.0, hss[i], touts[i], tgs[i])
+        assert got[i][0] == L, i
+        assert np.array_equal(got[i][1], gc) and np.array_equal(got[i][2], gy), i
+    with Integrator(olib, prob, y0s[0], atol=1e-10, rtol=1e-10) as o:
+        o.set_coef(coefs[0])
+        Lo, gco, gyo = o.loss_grad(y0s[0], 0.0, hss[0], touts[0], tgs[0])
+        assert got[0][0] == pytest.approx(Lo, rel=1e-12)
+        assert np.abs(got[0][1] - gco).max() <= 1e-10 * np.abs(gco).max()
+        assert np.abs(got[0][2] - gyo).max() <= 1e-10 * np.abs(gyo).max()
+    for it in its:
+        it.close()

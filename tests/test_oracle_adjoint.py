@@ -87,3 +87,54 @@ def test_adjoint_matches_central_differences(olib):
               it.loss_grad(ym, 0.0, hs, t_out, target, grad=False)) / (2 * eps)
         assert fd == pytest.approx(gy[i], rel=1e-6, abs=1e-6 * np.abs(gy).max()), i
     it.close()
+
+
+def _batch_instances(lib, prob, count, t1=0.3, nout=3):
+    """count instances of one problem with perturbed coefficients and their
+    own initial states, schedules and targets."""
+    rng = np.random.default_rng(11)
+    base = np.array([t.coef for t in prob.terms])
+    its, y0s, hss, touts, tgs = [], [], [], [], []
+    for i in range(count):
+        y0 = glibc_rand_ic(prob.n, prob.N, seed=20 + i, A=1.0)
+        it = Integrator(lib, prob, y0, atol=1e-10, rtol=1e-10)
+        it.set_coef(base * (1 + 0.05 * rng.standard_normal(base.size)))
+        hs = it.schedule(0.0, 0.01, t1)
+        it.set_state(y0)
+        t_out = np.linspace(t1 / nout, t1, nout)
+        its.append(it)
+        y0s.append(y0)
+        hss.append(hs)
+        touts.append(t_out)
+        tgs.append(y0[None, :] * 0.9 + 0.01 * rng.standard_normal((nout, y0.size)))
+    return its, y0s, hss, touts, tgs
+
+
+def test_batch_gradients_equal_single_calls(olib):
+    """rdcu_loss_grad_batch gives each instance what rdcu_loss_grad gives it."""
+    from rdcu_amd import loss_grad_batch
+    prob = CASES[1][1]
+    its, y0s, hss, touts, tgs = _batch_instances(olib, prob, 3)
+    got = loss_grad_batch(its, y0s, [0.0] * 3, hss, touts, tgs, concurrency=2)
+    for i, it in enumerate(its):
+        L, gc, gy = it.loss_grad(y0s[i], 0.0, hss[i], touts[i], tgs[i])
+        assert got[i][0] == L
+        assert np.array_equal(got[i][1], gc) and np.array_equal(got[i][2], gy)
+    losses = loss_grad_batch(its, y0s, [0.0] * 3, hss, touts, tgs, grad=False)
+    assert losses == [g[0] for g in got]
+    for it in its:
+        it.close()
+
+
+def test_batch_gradients_reject_shared_handles_and_report_failures(olib):
+    from rdcu_amd import RdcuError, loss_grad_batch
+    prob = CASES[1][1]
+    its, y0s, hss, touts, tgs = _batch_instances(olib, prob, 2)
+    with pytest.raises(ValueError):
+        loss_grad_batch([its[0], its[0]], y0s, [0.0] * 2, hss, touts, tgs)
+    bad = [touts[0], touts[1] + 100.0]  # second instance: output beyond its schedule
+    with pytest.raises(RdcuError) as e:
+        loss_grad_batch(its, y0s, [0.0] * 2, hss, bad, tgs)
+    assert e.value.code == 1 and "beyond" in str(e.value)
+    for it in its:
+        it.close()
