@@ -22,10 +22,13 @@
 #include <sys/time.h>
 #include <unistd.h>
 
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -240,31 +243,46 @@ void print_help(int n, int ndim) {
       for (int i = 0; i < n; i++) printf("%i: u%i_%i%i\n", l++, i, j, k);
 }
 
-}  // namespace
-
-int main(int argc, char *argv[]) {
+// One instance of the command line: everything main() sets up before the
+// integrator exists (rdcu.cu:599-1059, non-stiff parts).
+struct Instance {
   Driver d;
+  double t = 0, h = 0, t1 = 1e2, dt = 1e0, atl = 1e-8, rtl = 1e-8, hmin = 0, hmax = INFINITY;
+  int gpu = 0, fixed = 0, reload = 0;
+  std::vector<double> coef;
+  std::vector<int> tfield, nfac, fac;
+  rdcu_problem prob;
+  std::vector<double> yloc;
+};
+
+enum { PREP_OK = 0, PREP_FAIL = 1, PREP_USAGE = 2, PREP_HELP = 3 };
+
+// Parses one command line and prepares its instance: flags, coupling terms,
+// .out header, initial state (file, -R reload or glibc rand), output files
+// and output times.  argv[0] is the program name.
+int prepare(int argc, char *argv[], Instance &I) {
+  Driver &d = I.d;
   gettimeofday(&d.start, NULL);
 
   // defaults (rdcu.cu:603-607)
-  double t1 = 1e2, dt = 1e0, atl = 1e-8, rtl = 1e-8, A = 1E-3, A0 = 0.0;
-  double h = 0.0, hmin = 0, hmax = INFINITY;
-  int gpu = 0, seed = 1, fixed = 0, n = 1, ndim = 1, ndim2 = 0, verbose = 0, help = 1, dense = 1,
-      stiff = 0, reload = 0;
+  double A = 1E-3, A0 = 0.0;
+  int seed = 1, n = 1, ndim = 1, ndim2 = 0, verbose = 0, help = 1, dense = 1, stiff = 0;
   int Nsloc[3] = {128, 128, 128};
   double Lsloc[3] = {1.0, 1.0, 1.0};
   std::vector<TermSpec> terms;
   const char *filebase = nullptr;
 
-  // getopt loop (rdcu.cu:612-731)
+  // getopt loop (rdcu.cu:612-731); optind = 0 restarts glibc's scan for a
+  // new argument vector
+  optind = 0;
   int ch;
   while (optind < argc) {
     if ((ch = getopt(argc, argv, "hvFRSn:N:L:c:A:t:d:s:D:g:r:a:")) != -1) {
       switch (ch) {
         case 'h': help = 1; break;
         case 'v': verbose = 1; break;
-        case 'F': fixed = 1; break;
-        case 'R': reload = 1; break;
+        case 'F': I.fixed = 1; break;
+        case 'R': I.reload = 1; break;
         case 'S': stiff = 1; break;
         case 'n': if (optarg) n = atoi(optarg); break;
         case 'N':
@@ -288,23 +306,23 @@ int main(int argc, char *argv[]) {
             if (nAs > 1) A0 = As[1];
           }
           break;
-        case 't': if (optarg) t1 = atof(optarg); break;
+        case 't': if (optarg) I.t1 = atof(optarg); break;
         case 'd':
           if (optarg) {
             double dts[4];
             int ndts = 0;
             fparse_list(optarg, ",", dts, &ndts, 4);
-            dt = dts[0];
-            if (ndts > 1) h = dts[1];
-            if (ndts > 2) hmin = dts[2];
-            if (ndts > 3) hmax = dts[3];
+            I.dt = dts[0];
+            if (ndts > 1) I.h = dts[1];
+            if (ndts > 2) I.hmin = dts[2];
+            if (ndts > 3) I.hmax = dts[3];
           }
           break;
         case 's': if (optarg) seed = atoi(optarg); break;
         case 'D': if (optarg) dense = atoi(optarg); break;
-        case 'g': if (optarg) gpu = atoi(optarg); break;
-        case 'r': if (optarg) rtl = atof(optarg); break;
-        case 'a': if (optarg) atl = atof(optarg); break;
+        case 'g': if (optarg) I.gpu = atoi(optarg); break;
+        case 'r': if (optarg) I.rtl = atof(optarg); break;
+        case 'a': if (optarg) I.atl = atof(optarg); break;
       }
     } else {
       filebase = argv[optind];
@@ -315,17 +333,20 @@ int main(int argc, char *argv[]) {
 
   if (help) {
     print_help(n, ndim);
-    return 0;
+    return PREP_HELP;
   }
   if (stiff) {
     fprintf(stderr, "rdcu: the stiff BDF path (-S) is not part of this build\n");
-    return 2;
+    return PREP_USAGE;
   }
   if (dense >= 4) {
     fprintf(stderr, "rdcu: -D4 (CSR Jacobian output) needs the stiff path (-S), which is not "
                     "part of this build\n");
-    return 2;
+    return PREP_USAGE;
   }
+  const double t1 = I.t1, dt = I.dt;
+  const int fixed = I.fixed, reload = I.reload;
+  double &t = I.t, &h = I.h;
 
   // factor names (rdcu.cu:789-805)
   std::vector<std::string> sterms;
@@ -356,7 +377,7 @@ int main(int argc, char *argv[]) {
   d.out = fopen(file.c_str(), "w");
   if (!d.out) {
     fprintf(stderr, "rdcu: cannot open %s\n", file.c_str());
-    return 1;
+    return PREP_FAIL;
   }
   for (int i = 0; i < argc; i++) fprintf(d.out, "%s ", argv[i]);
   fprintf(d.out, "\n");
@@ -377,15 +398,15 @@ int main(int argc, char *argv[]) {
   const int nblk = 1 + ndim + ndim * ndim;
   const int nidx = n * nblk;
   // terms -> dictionary arrays; a trailing index without a power gets power 0
-  std::vector<double> coef;
-  std::vector<int> tfield, nfac, fac;
+  std::vector<double> &coef = I.coef;
+  std::vector<int> &tfield = I.tfield, &nfac = I.nfac, &fac = I.fac;
   for (const TermSpec &t : terms) {
     int fld = t.ints.empty() ? 0 : t.ints[0];
     int np = (int)t.ints.size();
     int nf = np / 2;
     if (fld < 0 || fld >= n) {
       fprintf(stderr, "rdcu: term field %d out of range (n=%d)\n", fld, n);
-      return 2;
+      return PREP_USAGE;
     }
     coef.push_back(t.C);
     tfield.push_back(fld);
@@ -395,7 +416,7 @@ int main(int argc, char *argv[]) {
       int pw = 2 + 2 * j < np ? t.ints[2 + 2 * j] : 0;
       if (ix < 0 || ix >= nidx) {
         fprintf(stderr, "rdcu: factor index %d out of range (%d indices)\n", ix, nidx);
-        return 2;
+        return PREP_USAGE;
       }
       fac.push_back(ix);
       fac.push_back(pw);
@@ -431,7 +452,7 @@ int main(int argc, char *argv[]) {
   fflush(stdout);
   fflush(d.out);
 
-  double t = 0;
+  t = 0;
   long N = Nsloc[0];
   for (int i = 1; i < ndim; i++) N *= Nsloc[i];
   const size_t nN = (size_t)n * N;
@@ -439,7 +460,8 @@ int main(int argc, char *argv[]) {
   srand(seed);
   if (h <= 0.0) h = fixed ? dt : dt / 100;  // rdcu.cu:917-924
 
-  std::vector<double> yloc(nN, 0.0);
+  std::vector<double> &yloc = I.yloc;
+  yloc.assign(nN, 0.0);
   // initial conditions (rdcu.cu:926-1008)
   file = d.filebase + "ic.dat";
   int loaded = 0;
@@ -508,7 +530,7 @@ int main(int argc, char *argv[]) {
   int n1 = int(t1 / dt) + 1;
   for (int nn = n0; nn < n1; nn++) d.t_eval.push_back(dt * nn);
 
-  rdcu_problem prob;
+  rdcu_problem &prob = I.prob;
   memset(&prob, 0, sizeof prob);
   prob.n = n;
   prob.ndim = ndim;
@@ -521,13 +543,6 @@ int main(int argc, char *argv[]) {
   prob.field = tfield.data();
   prob.nfac = nfac.data();
   prob.fac = fac.data();
-  int st = rdcu_create(&prob, atl, rtl, fixed, yloc.data(), gpu, &d.ctx);
-  if (st) {
-    printf("integrator initialization failed: %s\n", rdcu_last_error());
-    fprintf(d.out, "integrator initialization failed: %s\n", rdcu_last_error());
-    fclose(d.out);
-    return 1;
-  }
 
   d.n = n;
   d.ndim = ndim;
@@ -540,39 +555,229 @@ int main(int argc, char *argv[]) {
   d.yloc.assign(nN, 0.0);
   d.floc.assign(nN, 0.0);
   if (dense >= 3) d.Yloc.assign(2 * nN * nblk, 0.0);
+  return PREP_OK;
+}
 
-  // initial frame (rdcu.cu:1155-1191); keyed on the -R flag, as there
-  if (dense >= 1 && !reload) {
-    put(d.outstates, yloc.data(), sizeof(double), nN);
-    put(d.outtimes, &t, sizeof(double), 1);
+// Initial frame (rdcu.cu:1155-1191), once the instance has a handle; keyed on
+// the -R flag, as there.
+int first_frame(Instance &I) {
+  Driver &d = I.d;
+  const size_t nN = I.yloc.size();
+  if (d.dense >= 1 && !I.reload) {
+    put(d.outstates, I.yloc.data(), sizeof(double), nN);
+    put(d.outtimes, &I.t, sizeof(double), 1);
   }
-  if (dense >= 2 && !reload) {
-    st = rdcu_rhs(d.ctx, yloc.data(), d.floc.data(), dense >= 3 ? d.Yloc.data() : nullptr);
-    if (st) {
+  if (d.dense >= 2 && !I.reload) {
+    if (rdcu_rhs(d.ctx, I.yloc.data(), d.floc.data(), d.dense >= 3 ? d.Yloc.data() : nullptr)) {
       fprintf(stderr, "rdcu_rhs: %s\n", rdcu_last_error());
       return 1;
     }
     put(d.outf, d.floc.data(), sizeof(double), nN);
-    if (dense >= 3) put(d.outY, d.Yloc.data(), sizeof(double), d.Yloc.size());
+    if (d.dense >= 3) put(d.outY, d.Yloc.data(), sizeof(double), d.Yloc.size());
   }
+  return 0;
+}
 
-  double stop0 = d.t_eval.empty() ? INFINITY : d.t_eval[0];
-  st = rdcu_run(d.ctx, &t, &h, hmin, hmax, t1, stop0, &step_eval, &d);
-  int rc = 0;
-  if (st || d.failed) {
-    fprintf(stderr, "rdcu_run failed: %s\n", rdcu_last_error());
-    fprintf(d.out, "integration failed: %s\n", rdcu_last_error());
-    rc = 1;
-  }
-
+// Closing lines of the .out file (rdcu.cu:1265-1273) and the output files.
+void finish(Instance &I, bool failed) {
+  Driver &d = I.d;
+  if (failed) fprintf(d.out, "integration failed: %s\n", rdcu_last_error());
   double rt = elapsed(d.start);
   printf("\nruntime: %f\n", rt);
   fprintf(d.out, "\nsteps: %i\n", d.steps);
   fprintf(d.out, "runtime: %f\n", rt);
   fflush(d.out);
   fclose(d.out);
-  for (FILE *f : {d.outtimes, d.outsteps, d.outstates, d.outf, d.outY})
-    if (f) fclose(f);
+  d.out = nullptr;
+  for (FILE **f : {&d.outtimes, &d.outsteps, &d.outstates, &d.outf, &d.outY})
+    if (*f) {
+      fclose(*f);
+      *f = nullptr;
+    }
+}
+
+int run_single(Instance &I) {
+  Driver &d = I.d;
+  int st = rdcu_create(&I.prob, I.atl, I.rtl, I.fixed, I.yloc.data(), I.gpu, &d.ctx);
+  if (st) {
+    printf("integrator initialization failed: %s\n", rdcu_last_error());
+    fprintf(d.out, "integrator initialization failed: %s\n", rdcu_last_error());
+    fclose(d.out);
+    d.out = nullptr;
+    return 1;
+  }
+  if (first_frame(I)) return 1;
+  double stop0 = d.t_eval.empty() ? INFINITY : d.t_eval[0];
+  st = rdcu_run(d.ctx, &I.t, &I.h, I.hmin, I.hmax, I.t1, stop0, &step_eval, &d);
+  const bool failed = st || d.failed;
+  if (failed) fprintf(stderr, "rdcu_run failed: %s\n", rdcu_last_error());
+  finish(I, failed);
   rdcu_destroy(d.ctx);
+  d.ctx = nullptr;
+  return failed ? 1 : 0;
+}
+
+}  // namespace
+
+#ifndef RDCU_SWEEP
+int main(int argc, char *argv[]) {
+  Instance I;
+  const int p = prepare(argc, argv, I);
+  if (p == PREP_HELP) return 0;
+  if (p != PREP_OK) return p;
+  return run_single(I);
+}
+#endif
+
+#ifdef RDCU_SWEEP
+// rdcu_sweep MANIFEST -- a parameter sweep as batches instead of one process
+// per instance (the job array of sweep.sh:21-41, whose every task runs
+// makelines.py:260-268 -> ./rdcu).  Every non-empty line of MANIFEST ('#'
+// starts a comment) is the argument list of one rdcu run: the same flags and
+// FILEBASE, split at whitespace.  Each instance writes exactly the files that
+// rdcu would write for its line.  Instances that agree on -F, -r, -a, -g and
+// the step bounds of -d share one batch (rdcu_batch_*); a batch runs one
+// attempt per call, so every attempt's step size (steps.dat) and every
+// output time are seen as rdcu sees them.  A group the batch cannot hold
+// (3-D grids, lengths other than 2^k and 3*2^k) runs one instance at a time.
+namespace {
+
+std::vector<std::string> split_ws(const std::string &line) {
+  std::vector<std::string> out;
+  size_t i = 0;
+  while (i < line.size()) {
+    while (i < line.size() && isspace((unsigned char)line[i])) ++i;
+    size_t j = i;
+    while (j < line.size() && !isspace((unsigned char)line[j])) ++j;
+    if (j > i) out.push_back(line.substr(i, j - i));
+    i = j;
+  }
+  return out;
+}
+
+struct GroupKey {
+  int fixed, gpu;
+  double atl, rtl, hmin, hmax;
+  bool operator<(const GroupKey &o) const {
+    if (fixed != o.fixed) return fixed < o.fixed;
+    if (gpu != o.gpu) return gpu < o.gpu;
+    if (atl != o.atl) return atl < o.atl;
+    if (rtl != o.rtl) return rtl < o.rtl;
+    if (hmin != o.hmin) return hmin < o.hmin;
+    return hmax < o.hmax;
+  }
+};
+
+// Runs the instances of one group as a batch; 0 on success.
+int run_batch(std::vector<Instance *> &g) {
+  const int cnt = (int)g.size();
+  const Instance &g0 = *g[0];
+  std::vector<rdcu_problem> probs(cnt);
+  std::vector<const double *> y0s(cnt);
+  for (int k = 0; k < cnt; ++k) {
+    probs[k] = g[k]->prob;
+    y0s[k] = g[k]->yloc.data();
+  }
+  rdcu_batch *b = nullptr;
+  if (rdcu_batch_create(probs.data(), cnt, y0s.data(), g0.atl, g0.rtl, g0.fixed, g0.gpu, &b)) {
+    fprintf(stderr, "rdcu_sweep: %d instance(s) not batched (%s): running them one at a time\n",
+            cnt, rdcu_last_error());
+    int rc = 0;
+    for (Instance *I : g) rc |= run_single(*I);
+    return rc;
+  }
+  int rc = 0;
+  for (int k = 0; k < cnt && !rc; ++k) {
+    if (rdcu_batch_member(b, k, &g[k]->d.ctx) || first_frame(*g[k])) {
+      fprintf(stderr, "rdcu_sweep: %s\n", rdcu_last_error());
+      rc = 1;
+    }
+  }
+  std::vector<double> t(cnt), h(cnt), t1(cnt);
+  std::vector<long long> att(cnt), acc0(cnt);
+  for (int k = 0; k < cnt; ++k) {
+    t[k] = g[k]->t;
+    h[k] = g[k]->h;
+    t1[k] = g[k]->t1;
+  }
+  long long stats[4];
+  while (!rc) {
+    for (int k = 0; k < cnt; ++k) {
+      rdcu_get_stats(g[k]->d.ctx, stats);
+      acc0[k] = stats[1];
+    }
+    if (rdcu_batch_run(b, t.data(), h.data(), g0.hmin, g0.hmax, t1.data(), 1, att.data())) {
+      fprintf(stderr, "rdcu_batch_run failed: %s\n", rdcu_last_error());
+      rc = 1;
+      break;
+    }
+    bool any = false;
+    for (int k = 0; k < cnt; ++k) {
+      if (att[k] == 0) continue;
+      any = true;
+      rdcu_get_stats(g[k]->d.ctx, stats);
+      double stop;
+      step_eval(g[k]->d.ctx, t[k], h[k], stats[1] > acc0[k], 1, &stop, &g[k]->d);
+      if (g[k]->d.failed) rc = 1;
+    }
+    if (!any) break;
+  }
+  for (int k = 0; k < cnt; ++k) {
+    g[k]->t = t[k];
+    g[k]->h = h[k];
+    if (g[k]->d.out) finish(*g[k], rc != 0);
+  }
+  rdcu_batch_destroy(b);
   return rc;
 }
+
+}  // namespace
+
+int main(int argc, char *argv[]) {
+  if (argc != 2) {
+    fprintf(stderr, "usage: rdcu_sweep MANIFEST\n"
+                    "  every non-empty line of MANIFEST ('#' starts a comment) holds the\n"
+                    "  arguments of one rdcu run (flags and FILEBASE); the runs are\n"
+                    "  integrated side by side and each writes rdcu's files.\n");
+    return 2;
+  }
+  FILE *mf = fopen(argv[1], "r");
+  if (!mf) {
+    fprintf(stderr, "rdcu_sweep: cannot open %s\n", argv[1]);
+    return 1;
+  }
+  std::vector<std::unique_ptr<Instance>> inst;
+  std::string line;
+  int lineno = 0, c;
+  bool eof = false;
+  while (!eof) {
+    line.clear();
+    while ((c = fgetc(mf)) != EOF && c != '\n') line.push_back((char)c);
+    eof = c == EOF;
+    ++lineno;
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    std::vector<std::string> tok = split_ws(line);
+    if (tok.empty()) continue;
+    tok.insert(tok.begin(), "rdcu");
+    std::vector<char *> av;
+    for (auto &s : tok) av.push_back(&s[0]);
+    av.push_back(nullptr);
+    inst.push_back(std::make_unique<Instance>());
+    const int p = prepare((int)tok.size(), av.data(), *inst.back());
+    if (p != PREP_OK) {
+      fprintf(stderr, "rdcu_sweep: %s line %d: %s\n", argv[1], lineno,
+              p == PREP_HELP ? "no FILEBASE" : "not a runnable rdcu command line");
+      fclose(mf);
+      return p == PREP_HELP ? 2 : p;
+    }
+  }
+  fclose(mf);
+  std::map<GroupKey, std::vector<Instance *>> groups;
+  for (auto &I : inst)
+    groups[GroupKey{I->fixed, I->gpu, I->atl, I->rtl, I->hmin, I->hmax}].push_back(I.get());
+  int rc = 0;
+  for (auto &kv : groups) rc |= run_batch(kv.second);
+  return rc;
+}
+#endif

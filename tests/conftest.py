@@ -12,6 +12,8 @@ sys.path.insert(0, ROOT)
 ORACLE_LIB = os.path.join(ROOT, "oracle", "_bin", "liboracle.so")
 ORACLE_CLI = os.path.join(ROOT, "oracle", "_bin", "rdcu_cpu")
 HIP_CLI = os.path.join(ROOT, "rdcu_amd", "bin", "rdcu")
+ORACLE_SWEEP = os.path.join(ROOT, "oracle", "_bin", "rdcu_cpu_sweep")
+HIP_SWEEP = os.path.join(ROOT, "rdcu_amd", "bin", "rdcu_sweep")
 
 
 def pytest_configure(config):
