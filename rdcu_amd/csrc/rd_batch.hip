@@ -80,6 +80,27 @@ size_t rowi_lds(const BatchClass &cl, int G, int R) {
   return (size_t)R * rowi_region_b(cl, G) * sizeof(cx);
 }
 
+// Batch classes of these lengths run kernels compiled for the length (x lines:
+// half lengths of 64-, 128- and 256-point rows; strided lines of 64, 128 and
+// 256 points); other lengths run the run-time-plan kernels.
+constexpr int kRowLsB[] = {32, 64, 128};
+constexpr int kColLsB[] = {64, 128, 256};
+
+template <int E, int LS>
+void go_rowf_b(rdcu_batch *b, const BatchClass &cl, dim3 g, dim3 bl, size_t lds, int per,
+               int src_kind, const rd::StageCoef &sc, int rpb, int check) {
+  const rdcu_ctx *c = cl.c0;
+  hipLaunchKernelGGL((rd::k_rowf_b<E, LS>), g, bl, lds, b->stream, b->d_inst + cl.off, cl.cnt, per,
+                     c->px, c->twN, src_kind, sc, rpb, check);
+}
+
+template <int E, int LS>
+void go_col_b(rdcu_batch *b, const BatchClass &cl, dim3 g, dim3 bl, size_t lds, int per,
+              const rd::ColGeom &cg, int fpw, int check) {
+  hipLaunchKernelGGL((rd::k_col_b<E, LS>), g, bl, lds, b->stream, b->d_inst + cl.off, cl.cnt, per,
+                     cl.c0->p0, cg, 1, fpw, check);
+}
+
 template <int E>
 void launch_rowf_b(rdcu_batch *b, const BatchClass &cl, int src_kind, rd::StageCoef sc, int rpb,
                    long grid, int check) {
@@ -90,9 +111,13 @@ void launch_rowf_b(rdcu_batch *b, const BatchClass &cl, int src_kind, rd::StageC
   if (total == 0) return;
   const long g = grid > 0 ? std::min(grid, total) : total;
   const size_t lds = (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx);
-  hipLaunchKernelGGL(rd::k_rowf_b<E>, dim3((unsigned)g), dim3(rpb * T), lds, b->stream,
-                     b->d_inst + cl.off, cl.cnt, per, c->px, c->twN, src_kind,
-                     sc, rpb, check);
+  const dim3 gd((unsigned)g), bl(rpb * T);
+  if constexpr (E == 8) {
+    if (c->L == kRowLsB[0]) return go_rowf_b<E, kRowLsB[0]>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
+    if (c->L == kRowLsB[1]) return go_rowf_b<E, kRowLsB[1]>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
+    if (c->L == kRowLsB[2]) return go_rowf_b<E, kRowLsB[2]>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
+  }
+  go_rowf_b<E, 0>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
 }
 
 template <int E>
@@ -114,8 +139,13 @@ void launch_col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, int chec
   const long g = (total + 7) / 8 * 8;  // one item per block, a multiple of the 8 XCDs
   const size_t lds = (size_t)(cg.L + tc * rd::padded_len(cg.L) + tc * cg.L) * sizeof(cx) +
                      (size_t)cg.L * sizeof(double);
-  hipLaunchKernelGGL(rd::k_col_b<E>, dim3((unsigned)g), dim3(tc * T), lds, b->stream,
-                     b->d_inst + cl.off, cl.cnt, per, c->p0, cg, 1, fpw, check);
+  const dim3 gd((unsigned)g), bl(tc * T);
+  if constexpr (E == 8) {
+    if (cg.L == kColLsB[0]) return go_col_b<E, kColLsB[0]>(b, cl, gd, bl, lds, per, cg, fpw, check);
+    if (cg.L == kColLsB[1]) return go_col_b<E, kColLsB[1]>(b, cl, gd, bl, lds, per, cg, fpw, check);
+    if (cg.L == kColLsB[2]) return go_col_b<E, kColLsB[2]>(b, cl, gd, bl, lds, per, cg, fpw, check);
+  }
+  go_col_b<E, 0>(b, cl, gd, bl, lds, per, cg, fpw, check);
 }
 
 template <int E>
@@ -285,11 +315,21 @@ void build_jit_b(rdcu_batch *b) {
       const std::string tag = std::to_string(ci) + "_" + std::to_string(gi);
       src += jit_dict_source(m->n, m->n + m->nf_used, m->h_tfield, m->h_tstart, m->h_fslot,
                              m->h_fpow, "JitDict_" + tag);
+      // sizes shared by the group (one grid, one dictionary structure) as
+      // compile-time constants, as for a single problem's kernel
+      const rd::Grid &gd = m->grid;
+      char gbuf[512];
+      snprintf(gbuf, sizeof gbuf,
+               "namespace rd {\nstruct JitGeom_%s {\n  static constexpr int n = %d, ndim = %d, "
+               "Nx = %d, Nxp = %d, nf = %d;\n  static constexpr long N = %ldL, rows = %ldL;\n};\n}\n",
+               tag.c_str(), gd.n, gd.ndim, gd.Nx, gd.Nxp, m->fl_used.nf, gd.N, gd.rows);
+      src += gbuf;
       src += "extern \"C\" __global__ __launch_bounds__(256) void rdcu_rowi_b_" + tag +
              "(const rd::Inst *__restrict__ in, int ninst, int rows, rd::FftPlan px, const rd::cx "
              "*__restrict__ twN, int src_kind, int dst_kind, int dst_slot, int err, rd::ErrCoef "
              "ec, rd::StageCoef nx, int G, int check, long region) {\n";
-      src += "  rd::rowi_batch<" + std::to_string(cl.Ex) + ", rd::JitDict_" + tag +
+      src += "  rd::rowi_batch<" + std::to_string(cl.Ex) + ", rd::JitDict_" + tag + ", " +
+             std::to_string(m->L) + ", rd::JitGeom_" + tag +
              ">(in, ninst, rows, px, twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check, "
              "region);\n}\n";
     }
@@ -545,9 +585,13 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
     static bool lds_raised = false;
     if (!lds_raised) {
       const int lim = 160 * 1024;
-      for (const void *f : {(const void *)rd::k_rowf_b<8>, (const void *)rd::k_rowf_b<24>,
-                            (const void *)rd::k_col_b<8>, (const void *)rd::k_col_b<24>,
-                            (const void *)rd::k_rowi_b<8>, (const void *)rd::k_rowi_b<24>})
+      for (const void *f :
+           {(const void *)rd::k_rowf_b<8>, (const void *)rd::k_rowf_b<24>,
+            (const void *)rd::k_col_b<8>, (const void *)rd::k_col_b<24>,
+            (const void *)rd::k_rowi_b<8>, (const void *)rd::k_rowi_b<24>,
+            (const void *)rd::k_rowf_b<8, kRowLsB[0]>, (const void *)rd::k_rowf_b<8, kRowLsB[1]>,
+            (const void *)rd::k_rowf_b<8, kRowLsB[2]>, (const void *)rd::k_col_b<8, kColLsB[0]>,
+            (const void *)rd::k_col_b<8, kColLsB[1]>, (const void *)rd::k_col_b<8, kColLsB[2]>})
         HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
       if ((st = allow_big_lds())) return st;
       lds_raised = true;

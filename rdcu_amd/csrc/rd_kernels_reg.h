@@ -582,7 +582,10 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
     ks[0] = b.k[st->k1s];
     for (int j = 1; j < 6; ++j) ks[j] = b.k[j];
     const int half = Nx / 2;
-#pragma unroll 4
+    // items in flight per thread: 4 (24 values per thread have no registers
+    // left for more than one)
+    constexpr int AXU = E <= 8 ? 4 : 1;
+#pragma unroll AXU
     for (int w = tid; w < n * half; w += nt) {
       const int f = w / half, j2 = 2 * (w - f * half);
       const long off = f * N + r * Nx + j2;
@@ -694,7 +697,7 @@ __global__ __launch_bounds__(256) void k_rowi_reg(Grid g, FftPlan px, const cx *
 // a barrier between items keeps the previous item's LDS reads ahead of the
 // next item's LDS writes.
 
-template <int E>
+template <int E, int LS = 0>
 __global__ __launch_bounds__(256) void k_rowf_b(const Inst *__restrict__ in, int ninst, int per,
                                                 FftPlan px, const cx *__restrict__ twN,
                                                 int src_kind, StageCoef sc, int rpb, int check) {
@@ -706,7 +709,8 @@ __global__ __launch_bounds__(256) void k_rowf_b(const Inst *__restrict__ in, int
     if (check && !attempt_active(st)) continue;
     if (!first) __syncthreads();
     first = false;
-    rowf_body<E>(it - (long)i * per, in[i].g, px, twN, st, in[i].b, src_kind, nullptr, sc, rpb);
+    rowf_body<E, false, LS>(it - (long)i * per, in[i].g, px, twN, st, in[i].b, src_kind, nullptr,
+                            sc, rpb);
   }
 }
 
@@ -714,7 +718,7 @@ __global__ __launch_bounds__(256) void k_rowf_b(const Inst *__restrict__ in, int
 // column pass spills at 24 elements per thread).  The grid is a multiple of
 // 8 and items are ordered so that each XCD takes a contiguous range of
 // logical blocks: neighbouring column tiles of one instance share its L2.
-template <int E>
+template <int E, int LS = 0>
 __global__ __launch_bounds__(256) void k_col_b(const Inst *__restrict__ in, int ninst, int per,
                                                FftPlan p, ColGeom cg, int mode, int fpw,
                                                int check) {
@@ -723,13 +727,13 @@ __global__ __launch_bounds__(256) void k_col_b(const Inst *__restrict__ in, int 
   const int i = (int)(l / per);
   DevState *st = in[i].st;
   if (check && !attempt_active(st)) return;
-  col_body<E>((int)(l - (long)i * per), in[i].g, p, cg, in[i].b, in[i].fl, mode, fpw);
+  col_body<E, false, LS>((int)(l - (long)i * per), in[i].g, p, cg, in[i].b, in[i].fl, mode, fpw);
 }
 
 // The x-inverse pass takes one (instance, row) item per block: it is the
 // kernel with the most live registers, and a grid-stride loop around it
 // spills at 24 elements per thread.
-template <int E, class DE>
+template <int E, class DE, int LS = 0, class GK = RunGeom>
 __device__ __forceinline__ void rowi_batch(const Inst *__restrict__ in, int ninst, int rows,
                                            const FftPlan &px, const cx *__restrict__ twN,
                                            int src_kind, int dst_kind, int dst_slot, int err,
@@ -738,16 +742,16 @@ __device__ __forceinline__ void rowi_batch(const Inst *__restrict__ in, int nins
   extern __shared__ cx smem[];
   // row regions are sized for the largest factor count and program of the class
   if constexpr (E <= 8) {
-    const int gt = G * (px.L / E), R = blockDim.x / gt, sub = threadIdx.x / gt;
+    const int gt = G * ((LS > 0 ? LS : px.L) / E), R = blockDim.x / gt, sub = threadIdx.x / gt;
     const long it = (long)blockIdx.x * R + sub;  // R divides rows: one instance per block
     if (it >= (long)ninst * rows) return;
     const int i = (int)(it / rows);
     DevState *st = in[i].st;
     if (check && !attempt_active(st)) return;
     const Inst &I = in[i];
-    rowi_body<E, DE>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind, nullptr,
-                     dst_kind, dst_slot, nullptr, err, ec, nx, G, smem + sub * region,
-                     threadIdx.x - sub * gt, gt);
+    rowi_body<E, DE, LS, GK>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind,
+                             nullptr, dst_kind, dst_slot, nullptr, err, ec, nx, G,
+                             smem + sub * region, threadIdx.x - sub * gt, gt);
   } else {  // one row per block (see rowi_rows)
     const long it = blockIdx.x;
     if (it >= (long)ninst * rows) return;
@@ -755,8 +759,9 @@ __device__ __forceinline__ void rowi_batch(const Inst *__restrict__ in, int nins
     DevState *st = in[i].st;
     if (check && !attempt_active(st)) return;
     const Inst &I = in[i];
-    rowi_body<E, DE>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind, nullptr,
-                     dst_kind, dst_slot, nullptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x);
+    rowi_body<E, DE, LS, GK>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind,
+                             nullptr, dst_kind, dst_slot, nullptr, err, ec, nx, G, smem,
+                             threadIdx.x, blockDim.x);
   }
 }
 
