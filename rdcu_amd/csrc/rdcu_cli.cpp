@@ -24,10 +24,6 @@
 
 #include <cctype>
 #include <cmath>
-#include <condition_variable>
-#include <deque>
-#include <mutex>
-#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -110,10 +106,17 @@ struct Driver {
   int failed = 0;
 };
 
+void put(FILE *f, const void *p, size_t sz, size_t cnt) {
+  if (f && cnt) {
+    fwrite(p, sz, cnt, f);
+    fflush(f);
+  }
+}
+
 // FILEBASEfs.dat = state, t, h (the layout -R reads back, rdcu.cu:322-328).
 // Written to a temporary name and renamed over the old file, so a run killed
 // mid-write leaves the previous restart point intact.
-void write_fs_now(const std::string &filebase, const double *y, size_t nN, double t, double h) {
+void write_fs(const std::string &filebase, const double *y, size_t nN, double t, double h) {
   const std::string fn = filebase + "fs.dat", tmp = fn + ".tmp";
   FILE *fs = fopen(tmp.c_str(), "wb");
   if (!fs) return;
@@ -124,112 +127,6 @@ void write_fs_now(const std::string &filebase, const double *y, size_t nN, doubl
   if (ok && rename(tmp.c_str(), fn.c_str()) == 0) return;
   fprintf(stderr, "could not write %s\n", fn.c_str());
   remove(tmp.c_str());
-}
-
-// Background writer for the binary outputs (steps, times, states, f, Y and
-// fs.dat): one thread writes the submitted buffers in submission order, so
-// the integration does not wait for the file system (the reference writes
-// synchronously from its step callback, rdcu.cu:252-341).  The file contents
-// are those of synchronous writes.  At most kMaxQueued bytes wait; a
-// submission beyond that blocks until the thread catches up.
-class Writer {
- public:
-  Writer() : th_([this] { loop(); }) {}
-  ~Writer() {
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      stop_ = true;
-    }
-    cv_.notify_all();
-    th_.join();
-  }
-  void write(FILE *f, const void *p, size_t bytes) {
-    if (bytes >= kMaxQueued) {  // too large to copy: written here, after the queue
-      drain();
-      fwrite(p, 1, bytes, f);
-      fflush(f);
-      return;
-    }
-    Job j;
-    j.f = f;
-    j.data.assign(static_cast<const char *>(p), static_cast<const char *>(p) + bytes);
-    push(std::move(j));
-  }
-  void fs(const std::string &base, const double *y, size_t nN, double t, double h) {
-    Job j;
-    j.base = base;
-    j.data.assign(reinterpret_cast<const char *>(y), reinterpret_cast<const char *>(y + nN));
-    j.t = t;
-    j.h = h;
-    push(std::move(j));
-  }
-  // Returns once every submitted buffer is written.
-  void drain() {
-    std::unique_lock<std::mutex> lk(mu_);
-    cv_.wait(lk, [this] { return q_.empty() && !busy_; });
-  }
-
- private:
-  struct Job {
-    FILE *f = nullptr;  // null: an fs.dat job for `base`
-    std::string base;
-    std::vector<char> data;
-    double t = 0, h = 0;
-  };
-  static constexpr size_t kMaxQueued = size_t(1) << 30;
-  void push(Job &&j) {
-    std::unique_lock<std::mutex> lk(mu_);
-    cv_.wait(lk, [this] { return queued_ < kMaxQueued || q_.empty(); });
-    queued_ += j.data.size();
-    q_.push_back(std::move(j));
-    cv_.notify_all();
-  }
-  void loop() {
-    std::unique_lock<std::mutex> lk(mu_);
-    for (;;) {
-      cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
-      if (q_.empty()) return;  // stop requested and nothing left
-      Job j = std::move(q_.front());
-      q_.pop_front();
-      busy_ = true;
-      lk.unlock();
-      if (j.f) {
-        if (fwrite(j.data.data(), 1, j.data.size(), j.f) != j.data.size())
-          fprintf(stderr, "rdcu: short write of an output file\n");
-        fflush(j.f);
-      } else {
-        write_fs_now(j.base, reinterpret_cast<const double *>(j.data.data()),
-                     j.data.size() / sizeof(double), j.t, j.h);
-      }
-      lk.lock();
-      busy_ = false;
-      queued_ -= j.data.size();
-      cv_.notify_all();
-    }
-  }
-  std::mutex mu_;
-  std::condition_variable cv_;
-  std::deque<Job> q_;
-  size_t queued_ = 0;
-  bool busy_ = false, stop_ = false;
-  std::thread th_;
-};
-
-Writer *g_writer = nullptr;  // set by main for the process's lifetime
-
-void put(FILE *f, const void *p, size_t sz, size_t cnt) {
-  if (!f || !cnt) return;
-  if (g_writer) {
-    g_writer->write(f, p, sz * cnt);
-  } else {
-    fwrite(p, sz, cnt, f);
-    fflush(f);
-  }
-}
-
-void write_fs(const std::string &filebase, const double *y, size_t nN, double t, double h) {
-  if (g_writer) g_writer->fs(filebase, y, nN, t, h);
-  else write_fs_now(filebase, y, nN, t, h);
 }
 
 // Per-attempt output, as step_eval (rdcu.cu:252-341).
@@ -684,7 +581,6 @@ int first_frame(Instance &I) {
 // Closing lines of the .out file (rdcu.cu:1265-1273) and the output files.
 void finish(Instance &I, bool failed) {
   Driver &d = I.d;
-  if (g_writer) g_writer->drain();  // the output files are closed below
   if (failed) fprintf(d.out, "integration failed: %s\n", rdcu_last_error());
   double rt = elapsed(d.start);
   printf("\nruntime: %f\n", rt);
@@ -723,16 +619,8 @@ int run_single(Instance &I) {
 
 }  // namespace
 
-// RDCU_SYNC_WRITES=1 writes every output from the step callback itself.
-bool async_writes() {
-  const char *e = getenv("RDCU_SYNC_WRITES");
-  return !(e && *e && *e != '0');
-}
-
 #ifndef RDCU_SWEEP
 int main(int argc, char *argv[]) {
-  Writer writer;
-  if (async_writes()) g_writer = &writer;
   Instance I;
   const int p = prepare(argc, argv, I);
   if (p == PREP_HELP) return 0;
@@ -853,8 +741,6 @@ int main(int argc, char *argv[]) {
                     "  integrated side by side and each writes rdcu's files.\n");
     return 2;
   }
-  Writer writer;
-  if (async_writes()) g_writer = &writer;
   FILE *mf = fopen(argv[1], "r");
   if (!mf) {
     fprintf(stderr, "rdcu_sweep: cannot open %s\n", argv[1]);

@@ -143,22 +143,3 @@ def test_stiff_and_jacobian_output_rejected(tmp_path):
 def test_bad_factor_index_rejected(tmp_path):
     r = run(["-n", 2, "-N", 16, "-c", "1.0,0,99,1", "x"], tmp_path, rc=2)
     assert "out of range" in r.stderr
-
-
-def test_background_writer_matches_synchronous_writes(tmp_path):
-    """The output files written by the background writer thread equal those
-    written synchronously from the step callback (RDCU_SYNC_WRITES=1)."""
-    import os as _os
-    args = ["-n", 2, "-N", "32,48", "-L", "20,30", "-t", 1, "-d", 0.1, "-A", 1, "-D3"]
-    args += sum((["-c", c] for c in to_cli(cgle_terms(2))), [])
-    a, b = tmp_path / "a", tmp_path / "b"
-    a.mkdir()
-    b.mkdir()
-    run(args + ["w"], a)
-    env = dict(_os.environ, RDCU_SYNC_WRITES="1")
-    r = subprocess.run([ORACLE_CLI] + [str(x) for x in args + ["w"]], cwd=b, env=env,
-                       capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0, r.stderr
-    for suf in ("steps.dat", "times.dat", "states.dat", "f.dat", "Y.dat", "fs.dat"):
-        assert (a / ("w" + suf)).read_bytes() == (b / ("w" + suf)).read_bytes(), suf
-    assert (a / "wtimes.dat").stat().st_size == 8 * 11
