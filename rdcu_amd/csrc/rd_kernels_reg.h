@@ -411,10 +411,11 @@ __device__ __forceinline__ void c2r_pre(const Grid &g, const FacList &fl, const 
   }
 }
 
-// LDS of one row of the x-inverse pass (complex units): twiddles, G FFT
-// lines, (n+nf)*Nx values, the dictionary program.
+// LDS of one row of the x-inverse pass (complex units): G FFT lines,
+// (n+nf)*Nx values, the dictionary program.  The twiddles are read from
+// global memory (cached), which leaves more rows per CU.
 __host__ __device__ inline long rowi_region(int L, int Nx, int n, int nf, int nwords, int G) {
-  return (long)L + (long)G * padded_len(L) + ((long)(n + nf) * Nx + 1) / 2 + (nwords + 3) / 4;
+  return (long)G * padded_len(L) + ((long)(n + nf) * Nx + 1) / 2 + (nwords + 3) / 4;
 }
 
 // One grid row r, by the nt threads tid = 0..nt-1 of a row group whose LDS
@@ -434,7 +435,8 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   const int gl = tid / T, t = tid - gl * T;
   const int nf = GK::nf >= 0 ? GK::nf : fl.nf;
   const int PL = padded_len(L);
-  cx *tw = smem, *buf = smem + L;
+  const cx *tw = px.tw;
+  cx *buf = smem;
   double *vals = reinterpret_cast<double *>(buf + G * PL);
   // the stage results for the fused next stage share the FFT lines' LDS
   // (free between the factor transforms and the next stage's transform; the
@@ -448,14 +450,12 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   // use, so they share one memory round trip: for E = 8 the first factor
   // group's pre-processed spectrum (its addresses need no other load;
   // computed by every line group, without a divergent branch, and kept by
-  // those with a factor), the twiddles, the program and the identity rows.
+  // those with a factor), the program and the identity rows.
   constexpr bool kPre = E <= 8;  // register budget: only for 8 values per thread
   cx v0[kPre ? E : 1];
   if constexpr (kPre) {
     if (nf > 0) c2r_pre<E, LS, GK>(g, fl, b, twN, gl < nf ? gl : 0, r, t, T, v0);
   }
-  Prefetch<4, cx> pf_tw;
-  pf_tw.load(px.tw, L, tid, nt);
   Prefetch<4, int> pf_p;
   if constexpr (DE::kProg) pf_p.load(dict.prog, dict.nwords, tid, nt);
   constexpr int IDU = 8;
@@ -465,7 +465,6 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
     const int w = tid + u * nt;
     idr[u] = w < n * Nx ? src[(w / Nx) * N + r * Nx + (w % Nx)] : 0.0;
   }
-  pf_tw.store(tw, px.tw, L, tid, nt);
   if constexpr (DE::kProg) pf_p.store(prog, dict.prog, dict.nwords, tid, nt);
 #pragma unroll
   for (int u = 0; u < IDU; ++u) {
