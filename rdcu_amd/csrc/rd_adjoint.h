@@ -118,6 +118,17 @@ __global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *
     Gk[(long)(w / Nx) * N + r * Nx + (w % Nx)] = gacc[n * Nx + w];
 }
 
+// The seven stage values of the step just taken, into dst[j*nN + i]: every
+// step of a frozen schedule is accepted, so k1 and k7 have swapped roles
+// (K1 now in the k7 slot, K7 in the k1 slot); K2..K6 are slots 1..5.
+__global__ void k_save_stages(const DevState *st, Bufs b, double *dst, long nN) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nN) return;
+  dst[i] = b.k[st->k7s][i];
+  for (int j = 1; j <= 5; ++j) dst[j * nN + i] = b.k[j][i];
+  dst[6 * nN + i] = b.k[st->k1s][i];
+}
+
 // y_n of the current step (the buffer holding y after the FSAL rotation).
 __global__ void k_save_state(const DevState *st, Bufs b, double *dst, long nN) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;

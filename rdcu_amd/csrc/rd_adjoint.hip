@@ -12,6 +12,11 @@
 
 namespace {
 
+bool adj_store_enabled() {
+  const char *e = getenv("RDCU_ADJ_STORE");
+  return !(e && !strcmp(e, "0"));
+}
+
 // RAII list of device allocations of one call, ordered on the call's stream
 // (stream-ordered allocation: no device-wide synchronisation, so calls on
 // other streams keep running; plain hipMalloc where pools are unsupported).
@@ -210,6 +215,18 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
     return st;
   if (grad && ((st = ds.get(&ych, (size_t)ns * nN)) || (st = ds.get(&gbar, (size_t)nout * nN))))
     return st;
+  // The stage values of every step are kept from the forward pass when they
+  // fit in a quarter of the free device memory (RDCU_ADJ_STORE=0: never);
+  // otherwise the reverse sweep recomputes them from the checkpoints.  Both
+  // give the same bits.
+  double *kst = nullptr;
+  if (grad && adj_store_enabled()) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess &&
+        (double)7 * ns * nN * sizeof(double) <= 0.25 * (double)fr &&
+        (st = ds.get(&kst, (size_t)7 * ns * nN)))
+      return st;
+  }
   HIPCHECK(hipMemcpyAsync(d_hs, hs, ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemcpyAsync(d_tg, target, (size_t)nout * nN * sizeof(double), hipMemcpyHostToDevice,
                           c->stream));
@@ -230,6 +247,9 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
       hipLaunchKernelGGL(rd::k_save_state, dim3(blocks_of(nN)), dim3(256), 0, c->stream, c->st,
                          c->bufs, ych + (size_t)k * nN, nN);
     HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+    if (kst)
+      hipLaunchKernelGGL(rd::k_save_stages, dim3(blocks_of(nN)), dim3(256), 0, c->stream, c->st,
+                         c->bufs, kst + (size_t)k * 7 * nN, nN);
     for (; o < nout && ostep[o] == k; ++o) {
       hipLaunchKernelGGL(rd::k_dense, dim3(blocks_of(nN)), dim3(256), 0, c->stream, c->st, c->bufs,
                          dc, t_out[o], c->yeval, nN);
@@ -257,7 +277,8 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
   double *K[7], *Ys[7], *kbar[7], *ybar, *lam, *out_id, *Gk, *Dk, *P;
   int *tmap, *fmap;
   for (int j = 0; j < 7; ++j)
-    if ((st = ds.get(&K[j], nN)) || (st = ds.get(&Ys[j], nN)) || (st = ds.get(&kbar[j], nN)))
+    if ((!kst && (st = ds.get(&K[j], nN))) || (st = ds.get(&Ys[j], nN)) ||
+        (st = ds.get(&kbar[j], nN)))
       return st;
   if ((st = ds.get(&ybar, nN)) || (st = ds.get(&lam, nN)) || (st = ds.get(&out_id, nN)) ||
       (st = ds.get(&Gk, (size_t)nf * N)) || (st = ds.get(&Dk, (size_t)nf * N)) ||
@@ -280,8 +301,12 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
   for (long long k = ns - 1; k >= 0; --k) {
     const double h = hs[k];
     const double *y = ych + (size_t)k * nN;
-    // stages from the checkpoint (the forward's operations)
-    if ((st = enqueue_rhs(c, 0, rd::SRC_PTR, y, rd::DST_PTR, 0, K[0], 0, 0))) return st;
+    // stage inputs from the checkpoint (the forward's operations); the stage
+    // values stored by the forward pass, or recomputed
+    if (kst)
+      for (int j = 0; j < 7; ++j) K[j] = kst + ((size_t)k * 7 + j) * nN;
+    else if ((st = enqueue_rhs(c, 0, rd::SRC_PTR, y, rd::DST_PTR, 0, K[0], 0, 0)))
+      return st;
     for (int s = 2; s <= 7; ++s) {
       rd::StageIn si{};
       si.ns = s - 1;
@@ -291,10 +316,10 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
       }
       hipLaunchKernelGGL(rd::k_stage_in, dim3(blocks_of(nN)), dim3(256), 0, c->stream, y, si, h,
                          Ys[s - 1], nN);
-      if ((st = enqueue_rhs(c, 0, rd::SRC_PTR, Ys[s - 1], rd::DST_PTR, 0, K[s - 1], 0, 0)))
+      if (!kst && (st = enqueue_rhs(c, 0, rd::SRC_PTR, Ys[s - 1], rd::DST_PTR, 0, K[s - 1], 0, 0)))
         return st;
     }
-    c->nrhs += 7;
+    if (!kst) c->nrhs += 7;
     for (int j = 0; j < 7; ++j)
       HIPCHECK(hipMemsetAsync(kbar[j], 0, nN * sizeof(double), c->stream));
     HIPCHECK(hipMemsetAsync(ybar, 0, nN * sizeof(double), c->stream));

@@ -147,3 +147,20 @@ def test_batched_gradients_equal_single_calls(hlib, olib):
         assert np.abs(got[0][2] - gyo).max() <= 1e-10 * np.abs(gyo).max()
     for it in its:
         it.close()
+
+
+def test_stored_stages_equal_recomputed_stages(hlib, monkeypatch):
+    """The reverse sweep with the forward pass's stage values kept (the
+    default when they fit) and with them recomputed from the checkpoints
+    (RDCU_ADJ_STORE=0) gives the same bits."""
+    prob = Problem(2, [32, 64], [30.0, 50.0], cgle_terms(2, 2.0, 0.8))
+    y0 = glibc_rand_ic(2, prob.N, seed=8, A=1.0)
+    t_out = np.array([0.1, 0.25])
+    print("case stored vs recomputed stages", flush=True)
+    g, coef, target, hs = fit_problem(prob, y0, t_out, 0.25, hlib, 1e-10)
+    a = g.loss_grad(y0, 0.0, hs, t_out, target)
+    monkeypatch.setenv("RDCU_ADJ_STORE", "0")
+    b = g.loss_grad(y0, 0.0, hs, t_out, target)
+    g.close()
+    assert a[0] == b[0]
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
