@@ -18,6 +18,8 @@
 #ifndef RDCU_DP45_H
 #define RDCU_DP45_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -122,6 +124,21 @@ int rdcu_set_state(rdcu_ctx *ctx, const double *y);
  * (the reference leaves the anti-Hermitian Nyquist leakage there, and
  * uninitialised memory in block 0). */
 int rdcu_rhs(rdcu_ctx *ctx, const double *y, double *f_out, double *Y_out);
+
+/* One output frame of step_eval (rdcu.cu:252-341): the dense output at t_eval
+ * into y_out[n*N] (as rdcu_eval) and, when not NULL, the RHS at that state
+ * into f_out and its factor blocks into Y_out (as rdcu_rhs on y_out).  Same
+ * values, bit for bit, as those two calls; the state stays on the device in
+ * between, and Y's complex layout is assembled there.  Replaces the dp45_eval
+ * / cudaMemcpy / dydt / makeY sequence of rdcu.cu:270-291. */
+int rdcu_frame(rdcu_ctx *ctx, double t_eval, double *y_out, double *f_out, double *Y_out);
+
+/* Page-locked host memory (*out = NULL on failure) for the buffers of
+ * rdcu_frame, rdcu_eval, rdcu_rhs and rdcu_get_state: the copies into it run
+ * as DMA at full link speed.  The reference copies into pageable memory
+ * (rdcu.cu:277, 283, 288, 320).  The CPU build returns ordinary memory. */
+int rdcu_host_alloc(size_t bytes, void **out);
+int rdcu_host_free(void *p);
 
 /* ---- Batches: many independent problems integrated side by side ----------
  * The reference runs a parameter sweep as one process per instance (a SLURM

@@ -1321,6 +1321,25 @@ int rdcu_rhs(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
   return RDCU_OK;
 }
 
+// One output frame: the dense output, then the RHS at it (rdcu.cu:270-291).
+int rdcu_frame(rdcu_ctx *c, double t_eval, double *y_out, double *f_out, double *Y_out) {
+  if (!c || !y_out) return fail(RDCU_EINVAL, "null argument");
+  int st = rdcu_eval(c, t_eval, y_out);
+  if (st || (!f_out && !Y_out)) return st;
+  return rdcu_rhs(c, y_out, f_out, Y_out);
+}
+
+int rdcu_host_alloc(size_t bytes, void **out) {
+  if (!out) return fail(RDCU_EINVAL, "null argument");
+  *out = std::malloc(bytes ? bytes : 16);
+  return *out ? RDCU_OK : fail(RDCU_ENOMEM, "malloc failed");
+}
+
+int rdcu_host_free(void *p) {
+  std::free(p);
+  return RDCU_OK;
+}
+
 int rdcu_get_stats(rdcu_ctx *c, long long out[4]) {
   if (!c || !out) return fail(RDCU_EINVAL, "null argument");
   out[0] = c->attempts;

@@ -33,7 +33,7 @@ SYMBOLS = (
     "rdcu_last_error", "rdcu_backend",
     "rdcu_batch_create", "rdcu_batch_destroy", "rdcu_batch_step", "rdcu_batch_run",
     "rdcu_batch_member", "rdcu_set_coef", "rdcu_schedule", "rdcu_loss_grad",
-    "rdcu_loss_grad_batch",
+    "rdcu_loss_grad_batch", "rdcu_frame", "rdcu_host_alloc", "rdcu_host_free",
 )
 
 
@@ -93,6 +93,9 @@ def load(path: str) -> ctypes.CDLL:
     lib.rdcu_get_state.argtypes = [vp, _DP]
     lib.rdcu_set_state.argtypes = [vp, _DP]
     lib.rdcu_rhs.argtypes = [vp, _DP, _DP, _DP]
+    lib.rdcu_frame.argtypes = [vp, ctypes.c_double, _DP, _DP, _DP]
+    lib.rdcu_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
+    lib.rdcu_host_free.argtypes = [vp]
     lib.rdcu_get_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     lib.rdcu_batch_create.argtypes = [ctypes.POINTER(_Problem), ctypes.c_int,
                                       ctypes.POINTER(_DP), ctypes.c_double, ctypes.c_double,
@@ -309,6 +312,19 @@ class Integrator:
         if factors:
             return f, Y.view(np.complex128).reshape(self.prob.nblk, self.prob.n, self.prob.N)
         return f
+
+    def frame(self, t_eval: float, dense: int = 3):
+        """One output frame at t_eval, as the CLI writes it at -D<dense>:
+        (y, f or None, Y or None), Y complex [nblk, n, N]."""
+        y = np.empty(self.nN)
+        f = np.empty(self.nN) if dense >= 2 else None
+        Y = np.empty(2 * self.nN * self.prob.nblk) if dense >= 3 else None
+        self._check(self.lib.rdcu_frame(self.h, t_eval, _ptr(y),
+                                        _ptr(f) if f is not None else None,
+                                        _ptr(Y) if Y is not None else None))
+        if Y is not None:
+            Y = Y.view(np.complex128).reshape(self.prob.nblk, self.prob.n, self.prob.N)
+        return y, f, Y
 
     def set_coef(self, coef):
         """Replaces the dictionary coefficients (term order)."""

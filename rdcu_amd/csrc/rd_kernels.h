@@ -691,4 +691,18 @@ __global__ void k_dense(const DevState *st, Bufs b, DenseCoef dc, double t_eval,
   out[i] = __builtin_fma(hh, a, b.y[st->iyl][i]);
 }
 
+// The factor blocks in the layout of FILEBASEY.dat (rdcu.cu:339-340): complex
+// out[(blk*n + field)*N + x] = (value, 0).  Block 0 is the state y itself;
+// block blk >= 1 of `field` is row field*(nblk-1) + blk-1 of Yr (the real-space
+// factor list in field-major order).  Grid: (ceil(N/blockDim), nblk*n).
+__global__ void k_pack_y(const double *__restrict__ y, const double *__restrict__ Yr,
+                         double2 *__restrict__ out, int n, int nblk, long N) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  const int bf = blockIdx.y, blk = bf / n, f = bf - blk * n;
+  const double v =
+      blk == 0 ? y[(long)f * N + x] : Yr[((long)f * (nblk - 1) + blk - 1) * N + x];
+  out[(long)bf * N + x] = make_double2(v, 0.0);
+}
+
 }  // namespace rd

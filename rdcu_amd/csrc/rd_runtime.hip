@@ -139,6 +139,7 @@ struct rdcu_ctx {
   rd::DevState *st = nullptr;
   rd::DevState *sh = nullptr;  // pinned host shadow
   double *rhs_in = nullptr, *rhs_out = nullptr, *Yreal = nullptr, *yeval = nullptr;
+  double *Ycx = nullptr;  // factor blocks as complex values (FILEBASEY.dat layout)
   std::vector<void *> allocs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1498,40 +1499,75 @@ int rdcu_set_state(rdcu_ctx *c, const double *y) {
   return RDCU_OK;
 }
 
-int rdcu_rhs(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
-  if (!c || !y) return fail(RDCU_EINVAL, "null argument");
-  HIPCHECK(hipSetDevice(c->dev));
+// RHS of the device state y_dev into c->rhs_out and, if Y_out is wanted,
+// every factor block packed as complex values into c->Ycx; then the copies
+// to the host (asynchronous on the library's stream; the caller syncs).
+int enqueue_rhs_out(rdcu_ctx *c, const double *y_dev, double *f_out, double *Y_out) {
   const long N = c->N, nN = (long)c->n * N;
-  HIPCHECK(hipMemcpyAsync(c->rhs_in, y, nN * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  int st = enqueue_rhs(c, 0, rd::SRC_PTR, c->rhs_in, rd::DST_PTR, 0, c->rhs_out, 0, 0);
+  int st = enqueue_rhs(c, 0, rd::SRC_PTR, y_dev, rd::DST_PTR, 0, c->rhs_out, 0, 0);
   if (st) return st;
-  if (Y_out) {
-    if (!c->lds_y) return fail(RDCU_EUNSUPPORTED, "factor output not available for lines this long");
-    if (!c->Yreal && (st = c->dalloc(&c->Yreal, (size_t)std::max(1, c->nf_all) * N))) return st;
-    if ((st = enqueue_all_factors(c))) return st;
-  }
   if (f_out)
     HIPCHECK(hipMemcpyAsync(f_out, c->rhs_out, nN * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  std::vector<double> yr;
-  if (Y_out && c->nf_all) {
-    yr.resize((size_t)c->nf_all * N);
-    HIPCHECK(hipMemcpyAsync(yr.data(), c->Yreal, yr.size() * sizeof(double), hipMemcpyDeviceToHost,
-                            c->stream));
+  if (Y_out) {
+    if (!c->Yreal && (st = c->dalloc(&c->Yreal, (size_t)std::max(1, c->nf_all) * N))) return st;
+    const size_t ny = 2 * (size_t)c->nblk * nN;
+    if (!c->Ycx && (st = c->dalloc(&c->Ycx, ny))) return st;
+    if ((st = enqueue_all_factors(c))) return st;
+    hipLaunchKernelGGL(rd::k_pack_y, dim3((unsigned)((N + 255) / 256), (unsigned)(c->nblk * c->n)),
+                       dim3(256), 0, c->stream, y_dev, c->Yreal, (double2 *)c->Ycx, c->n, c->nblk,
+                       N);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(Y_out, c->Ycx, ny * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  return RDCU_OK;
+}
+
+int rdcu_rhs(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
+  if (!c || !y) return fail(RDCU_EINVAL, "null argument");
+  if (Y_out && !c->lds_y) return fail(RDCU_EUNSUPPORTED, "factor output not available for lines this long");
+  HIPCHECK(hipSetDevice(c->dev));
+  const long nN = (long)c->n * c->N;
+  HIPCHECK(hipMemcpyAsync(c->rhs_in, y, nN * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int st = enqueue_rhs_out(c, c->rhs_in, f_out, Y_out);
+  if (st) return st;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return RDCU_OK;
+}
+
+int rdcu_frame(rdcu_ctx *c, double t_eval, double *y_out, double *f_out, double *Y_out) {
+  if (!c || !y_out) return fail(RDCU_EINVAL, "null argument");
+  if (!c->sh->eval_valid) return fail(RDCU_ESTATE, "no accepted step to interpolate");
+  if (Y_out && !c->lds_y) return fail(RDCU_EUNSUPPORTED, "factor output not available for lines this long");
+  HIPCHECK(hipSetDevice(c->dev));
+  rd::DenseCoef dc;
+  for (int j = 0; j < 7; ++j)
+    for (int q = 0; q < 4; ++q) dc.p[j][q] = PP[j][q];
+  const long nN = (long)c->n * c->N;
+  hipLaunchKernelGGL(rd::k_dense, dim3((unsigned)((nN + 255) / 256)), dim3(256), 0, c->stream,
+                     c->st, c->bufs, dc, t_eval, c->yeval, nN);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(y_out, c->yeval, nN * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (f_out || Y_out) {
+    int st = enqueue_rhs_out(c, c->yeval, f_out, Y_out);
+    if (st) return st;
   }
   HIPCHECK(hipStreamSynchronize(c->stream));
-  if (Y_out) {
-    for (int blk = 0; blk < c->nblk; ++blk)
-      for (int f = 0; f < c->n; ++f) {
-        double *dst = Y_out + 2 * ((size_t)(blk * c->n + f) * N);
-        // factor list "all" is field-major: factor of (f, blk) is f*(nblk-1) + blk-1
-        const double *src = blk == 0 ? y + (size_t)f * N
-                                     : yr.data() + (size_t)(f * (c->nblk - 1) + blk - 1) * N;
-        for (long x = 0; x < N; ++x) {
-          dst[2 * x] = src[x];
-          dst[2 * x + 1] = 0.0;
-        }
-      }
+  return RDCU_OK;
+}
+
+int rdcu_host_alloc(size_t bytes, void **out) {
+  if (!out) return fail(RDCU_EINVAL, "null argument");
+  *out = nullptr;
+  hipError_t e = hipHostMalloc(out, bytes ? bytes : 16, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return fail(RDCU_ENOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
   }
+  return RDCU_OK;
+}
+
+int rdcu_host_free(void *p) {
+  if (p) HIPCHECK(hipHostFree(p));
   return RDCU_OK;
 }
 

@@ -102,9 +102,48 @@ struct Driver {
   timeval start;
   FILE *out = nullptr, *outsteps = nullptr, *outtimes = nullptr, *outstates = nullptr,
        *outf = nullptr, *outY = nullptr;
-  std::vector<double> yloc, floc, Yloc;
   int failed = 0;
 };
+
+// Host buffers of one output frame (state, RHS, factor blocks), shared by
+// every instance of the process: frames are written one at a time.  They are
+// page-locked (rdcu_host_alloc), so the device copies into them run as DMA,
+// and grow on demand.  release_frames() frees them before exit.
+double *g_frame[3] = {nullptr, nullptr, nullptr};
+size_t g_frame_cap[3] = {0, 0, 0};
+
+double *frame_buf(int i, size_t count) {
+  if (count > g_frame_cap[i]) {
+    rdcu_host_free(g_frame[i]);
+    g_frame[i] = nullptr;
+    g_frame_cap[i] = 0;
+    void *q = nullptr;
+    if (rdcu_host_alloc(count * sizeof(double), &q)) return nullptr;
+    g_frame[i] = static_cast<double *>(q);
+    g_frame_cap[i] = count;
+  }
+  return g_frame[i];
+}
+
+void release_frames() {
+  for (int i = 0; i < 3; ++i) {
+    rdcu_host_free(g_frame[i]);
+    g_frame[i] = nullptr;
+    g_frame_cap[i] = 0;
+  }
+}
+
+// The frame buffers one output of `d` needs (NULL where its -D level writes
+// nothing); false if an allocation failed.
+bool frame_bufs(const Driver &d, double **y, double **f, double **Y) {
+  const size_t nN = (size_t)d.n * d.N;
+  *y = frame_buf(0, nN);
+  *f = d.dense >= 2 ? frame_buf(1, nN) : nullptr;
+  *Y = d.dense >= 3 ? frame_buf(2, 2 * nN * d.nblk) : nullptr;
+  if (*y && (d.dense < 2 || *f) && (d.dense < 3 || *Y)) return true;
+  fprintf(stderr, "frame buffers: %s\n", rdcu_last_error());
+  return false;
+}
 
 void put(FILE *f, const void *p, size_t sz, size_t cnt) {
   if (f && cnt) {
@@ -145,21 +184,19 @@ int step_eval(rdcu_ctx *ctx, double t, double h, int success, int current, doubl
       return 1;
     }
     const double te = d.t_eval[d.eval_i];
-    if (rdcu_eval(ctx, te, d.yloc.data())) {
-      fprintf(stderr, "rdcu_eval: %s\n", rdcu_last_error());
+    double *yb, *fb, *Yb;
+    if (!frame_bufs(d, &yb, &fb, &Yb)) {
       d.failed = 1;
       return 1;
     }
-    if (d.dense >= 1) put(d.outstates, d.yloc.data(), sizeof(double), nN);
-    if (d.dense >= 2) {
-      if (rdcu_rhs(ctx, d.yloc.data(), d.floc.data(), d.dense >= 3 ? d.Yloc.data() : nullptr)) {
-        fprintf(stderr, "rdcu_rhs: %s\n", rdcu_last_error());
-        d.failed = 1;
-        return 1;
-      }
-      put(d.outf, d.floc.data(), sizeof(double), nN);
-      if (d.dense >= 3) put(d.outY, d.Yloc.data(), sizeof(double), d.Yloc.size());
+    if (rdcu_frame(ctx, te, yb, fb, Yb)) {
+      fprintf(stderr, "rdcu_frame: %s\n", rdcu_last_error());
+      d.failed = 1;
+      return 1;
     }
+    if (d.dense >= 1) put(d.outstates, yb, sizeof(double), nN);
+    if (d.dense >= 2) put(d.outf, fb, sizeof(double), nN);
+    if (d.dense >= 3) put(d.outY, Yb, sizeof(double), 2 * nN * d.nblk);
     put(d.outtimes, &te, sizeof(double), 1);
     d.eval_i++;
     eval = 1;
@@ -175,12 +212,13 @@ int step_eval(rdcu_ctx *ctx, double t, double h, int success, int current, doubl
     }
     fprintf(d.out, "%.3f\t%1.3e\t%1.3e\t%f\t%i\t\n", frac, el, eta, h, d.steps);
     fflush(d.out);
-    if (rdcu_get_state(ctx, d.yloc.data())) {
+    double *yb = frame_buf(0, nN);
+    if (!yb || rdcu_get_state(ctx, yb)) {
       fprintf(stderr, "rdcu_get_state: %s\n", rdcu_last_error());
       d.failed = 1;
       return 1;
     }
-    write_fs(d.filebase, d.yloc.data(), nN, t, h);
+    write_fs(d.filebase, yb, nN, t, h);
   }
   return 0;
 }
@@ -552,9 +590,6 @@ int prepare(int argc, char *argv[], Instance &I) {
   d.verbose = verbose;
   d.t0 = t;
   d.t1 = t1;
-  d.yloc.assign(nN, 0.0);
-  d.floc.assign(nN, 0.0);
-  if (dense >= 3) d.Yloc.assign(2 * nN * nblk, 0.0);
   return PREP_OK;
 }
 
@@ -568,12 +603,14 @@ int first_frame(Instance &I) {
     put(d.outtimes, &I.t, sizeof(double), 1);
   }
   if (d.dense >= 2 && !I.reload) {
-    if (rdcu_rhs(d.ctx, I.yloc.data(), d.floc.data(), d.dense >= 3 ? d.Yloc.data() : nullptr)) {
+    double *yb, *fb, *Yb;
+    if (!frame_bufs(d, &yb, &fb, &Yb)) return 1;
+    if (rdcu_rhs(d.ctx, I.yloc.data(), fb, Yb)) {
       fprintf(stderr, "rdcu_rhs: %s\n", rdcu_last_error());
       return 1;
     }
-    put(d.outf, d.floc.data(), sizeof(double), nN);
-    if (d.dense >= 3) put(d.outY, d.Yloc.data(), sizeof(double), d.Yloc.size());
+    put(d.outf, fb, sizeof(double), nN);
+    if (d.dense >= 3) put(d.outY, Yb, sizeof(double), 2 * nN * d.nblk);
   }
   return 0;
 }
@@ -625,7 +662,9 @@ int main(int argc, char *argv[]) {
   const int p = prepare(argc, argv, I);
   if (p == PREP_HELP) return 0;
   if (p != PREP_OK) return p;
-  return run_single(I);
+  const int rc = run_single(I);
+  release_frames();
+  return rc;
 }
 #endif
 
@@ -778,6 +817,7 @@ int main(int argc, char *argv[]) {
     groups[GroupKey{I->fixed, I->gpu, I->atl, I->rtl, I->hmin, I->hmax}].push_back(I.get());
   int rc = 0;
   for (auto &kv : groups) rc |= run_batch(kv.second);
+  release_frames();
   return rc;
 }
 #endif

@@ -74,9 +74,38 @@ def test_error_codes_on_the_oracle(olib):
     p = Problem(2, [16], [8.0], cgle_terms(1))
     with Integrator(olib, p, np.full(32, 0.1)) as it:
         assert code_of(lambda: it.eval(0.5)) == ESTATE  # no accepted step yet
+        assert code_of(lambda: it.frame(0.5)) == ESTATE
         hs = np.full(4, 0.01)
         tgt = np.zeros(32)
         assert code_of(lambda: it.loss_grad(np.zeros(32), 0.0, hs, [1.0], tgt)) == EINVAL
         assert code_of(lambda: it.loss_grad(np.zeros(32), 0.0, hs, [0.0], tgt)) == EINVAL
         with _pt.raises(ValueError):
             it.set_coef([1.0])
+
+
+def test_oracle_frame_is_eval_then_rhs(olib):
+    """rdcu_frame on the oracle: the dense output, then the RHS and factor
+    blocks at it, at each -D level (the CLI's output path)."""
+    import numpy as np
+    from rdcu_amd import Integrator, Problem, cgle_terms, glibc_rand_ic
+    prob = Problem(2, [32, 16], [20.0, 10.0], cgle_terms(2))
+    with Integrator(olib, prob, glibc_rand_ic(2, 512, seed=2, A=1.0)) as it:
+        t, h = it.run(0.0, 0.01, 0.05)
+        te = 0.05
+        y = it.eval(te)
+        f, Y = it.rhs(y, factors=True)
+        for dense in (1, 2, 3):
+            y2, f2, Y2 = it.frame(te, dense)
+            assert np.array_equal(y2, y)
+            assert (f2 is None) == (dense < 2) and (Y2 is None) == (dense < 3)
+            if dense >= 2:
+                assert np.array_equal(f2, f)
+            if dense >= 3:
+                assert Y2.shape == (7, 2, 512) and np.array_equal(Y2, Y)
+
+
+def test_host_alloc_round_trip(olib):
+    vp = ctypes.c_void_p()
+    assert olib.rdcu_host_alloc(4096, ctypes.byref(vp)) == 0 and vp.value
+    ctypes.memset(vp, 0, 4096)
+    assert olib.rdcu_host_free(vp) == 0

@@ -194,12 +194,19 @@ def _run_cli(exe, args, cwd):
     return r
 
 
-def test_cli_outputs_identical(tmp_path):
+CLI_CASES = [(["-N", "256", "-L", "100", "-t", "5", "-d", "0.5"], 1),
+             (["-N", "64,96", "-L", "40,70", "-t", "1", "-d", "0.25"], 2),
+             (["-N", "16,24,32", "-L", "10,20,30", "-t", "0.2", "-d", "0.1"], 3)]
+
+
+@pytest.mark.parametrize("grid,nd", CLI_CASES, ids=["1d", "2d", "3d"])
+def test_cli_outputs_identical(tmp_path, grid, nd):
     """The drop-in CLI on the GPU writes byte-identical output files to the
-    same CLI on the oracle (1-D CGLE, -D3)."""
+    same CLI on the oracle (CGLE, -D3: states, f and Y frames through
+    rdcu_frame, restart file, steps and times)."""
     from conftest import HIP_CLI, ORACLE_CLI
-    args = ["-n", "2", "-N", "256", "-L", "100", "-t", "5", "-d", "0.5", "-A", "1", "-D3"]
-    args += sum((["-c", c] for c in __import__("rdcu_amd").to_cli(cgle_terms(1))), [])
+    args = ["-n", "2"] + grid + ["-A", "1", "-D3"]
+    args += sum((["-c", c] for c in __import__("rdcu_amd").to_cli(cgle_terms(nd))), [])
     for sub, exe in (("g", HIP_CLI), ("c", ORACLE_CLI)):
         d = tmp_path / sub
         d.mkdir()
@@ -208,6 +215,39 @@ def test_cli_outputs_identical(tmp_path):
         a = (tmp_path / "g" / ("run" + suf)).read_bytes()
         b = (tmp_path / "c" / ("run" + suf)).read_bytes()
         assert a == b, suf
+
+
+# (the mixed dictionary divides by u1, so its steps are kept tiny)
+FRAME_CASES = [([256], [100.0], "cgle", 0.01, 0.05), ([64, 96], [40.0, 70.0], "mixed", 1e-6, 3e-6),
+               ([16, 24, 32], [10.0, 20.0, 30.0], "mixed", 1e-6, 3e-6),
+               ([64, 64], [50.0, 50.0], "cgle", 0.01, 0.05)]
+
+
+@pytest.mark.parametrize("Ns,Ls,kind,h0,t1", FRAME_CASES, ids=["1d", "2d", "3d", "2d-cgle"])
+def test_frame_bitwise(Ns, Ls, kind, h0, t1, hlib, olib):
+    """rdcu_frame (state kept on the device, Y packed there) gives the bits of
+    rdcu_eval then rdcu_rhs, and the oracle's frame, at every -D level."""
+    prob = make_problem(Ns, Ls, kind)
+    N = int(np.prod(Ns))
+    y0 = glibc_rand_ic(2, N, seed=5, A=1.0)
+    out = {}
+    for name, lib in (("gpu", hlib), ("cpu", olib)):
+        with Integrator(lib, prob, y0) as it:
+            t, h = it.run(0.0, h0, t1)
+            y = it.eval(t1)
+            f, Y = it.rhs(y, factors=True)
+            for dense in (3, 1, 2):
+                y2, f2, Y2 = it.frame(t1, dense)
+                assert np.array_equal(y2, y)
+                if dense >= 2:
+                    assert np.array_equal(f2, f)
+                if dense >= 3:
+                    assert np.array_equal(Y2, Y)
+            out[name] = (t, h, y, f, Y)
+    g, c = out["gpu"], out["cpu"]
+    assert g[:2] == c[:2]
+    for a, b in zip(g[2:], c[2:]):
+        assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("Ns,Ls,shapes", [([512, 512], [200.0, 200.0], "2,8,1,4,1,16"),
