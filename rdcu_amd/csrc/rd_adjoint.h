@@ -137,19 +137,43 @@ __global__ void k_save_state(const DevState *st, Bufs b, double *dst, long nN) {
 
 // Stage input y + h*(sum_j a_j k_j), zero coefficients skipped (the
 // operations of the forward stage sums).
+// The reverse sweep replays one captured step for every step k: k is read
+// from the device (*kp, counted down by k_adj_next), and so are the values
+// that depend on it (the step size hs[k], the checkpoint, the stored stages,
+// the outputs that fall in the step).
+
+// Stage input s of step k: out = y + h*sum_j a_j K_j, K_j at si.k[j] + k*kstride
+// (kstride 0: recomputed stages in fixed buffers), h = hs[k] (dp45.cu:34-69).
 struct StageIn {
   const double *k[6];
   double a[6];
   int ns;
 };
-__global__ void k_stage_in(const double *__restrict__ y, StageIn si, double h, double *out,
-                           long nN) {
+__global__ void k_stage_in(const double *__restrict__ y, StageIn si, const double *__restrict__ hs,
+                           const long long *__restrict__ kp, long kstride, double *out, long nN) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nN) return;
-  double sacc = si.a[0] * si.k[0][i];
+  const long long k = *kp;
+  const long off = (long)k * kstride + i;
+  double sacc = si.a[0] * si.k[0][off];
   for (int j = 1; j < si.ns; ++j)
-    if (si.a[j] != 0.0) sacc = __builtin_fma(si.a[j], si.k[j][i], sacc);
-  out[i] = __builtin_fma(h, sacc, y[i]);
+    if (si.a[j] != 0.0) sacc = __builtin_fma(si.a[j], si.k[j][off], sacc);
+  out[i] = __builtin_fma(hs[k], sacc, y[i]);
+}
+
+// ycur = checkpoint (state at the start) of step k.
+__global__ void k_adj_fetch(const double *__restrict__ ych, const long long *__restrict__ kp,
+                            double *__restrict__ ycur, long nN) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nN) ycur[i] = ych[(long)*kp * nN + i];
+}
+
+__global__ void k_adj_setk(long long *kp, long long k) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *kp = k;
+}
+
+__global__ void k_adj_next(long long *kp) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *kp = *kp - 1;
 }
 
 // Loss terms of one output: gbar = 2 (y - target) / cnt; per-block sums of
@@ -178,23 +202,44 @@ struct AdjK {
   double c[7];  // per-array coefficient (0: skip)
 };
 
-// Adjoint of a dense output: ybar += g, kbar_j += c_j g.
-__global__ void k_adj_dense(double *ybar, AdjK kb, const double *__restrict__ g, long nN) {
+// Zeroes the stage adjoints kbar_0..6 (kb.k) and ybar.
+__global__ void k_adj_zero(AdjK kb, double *ybar, long nN) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nN) return;
-  const double gv = g[i];
-  ybar[i] += gv;
-  for (int j = 0; j < 7; ++j)
-    if (kb.c[j] != 0.0) kb.k[j][i] += kb.c[j] * gv;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) kb.k[j][i] = 0.0;
+  ybar[i] = 0.0;
+}
+
+// Adjoints of the dense outputs o of step k (obeg[k] <= o < oend[k], last
+// output first): ybar += g_o, kbar_j += c_oj g_o, with g_o at g + o*nN and
+// c_oj = oc[7*o + j] (0: skip).
+__global__ void k_adj_dense(double *ybar, AdjK kb, const double *__restrict__ g,
+                            const int *__restrict__ obeg, const int *__restrict__ oend,
+                            const double *__restrict__ oc, const long long *__restrict__ kp,
+                            long nN) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nN) return;
+  const long long k = *kp;
+  for (int o = oend[k] - 1; o >= obeg[k]; --o) {
+    const double gv = g[(long)o * nN + i];
+    ybar[i] += gv;
+    for (int j = 0; j < 7; ++j) {
+      const double cj = oc[7 * o + j];
+      if (cj != 0.0) kb.k[j][i] += cj * gv;
+    }
+  }
 }
 
 // Adjoint of stage s: Yb = out_id + sum_{m: fmap[m] = field} Dk[m] (+ lam),
-// ybar += Yb, kbar_j += c_j Yb.
+// ybar += Yb, kbar_j += (hs[k] a_j) Yb with a_j = kb.c[j] (0: skip).
 __global__ void k_adj_stage(const double *__restrict__ out_id, const double *__restrict__ Dk,
                             const int *__restrict__ fmap, int nf, const double *__restrict__ lam,
-                            double *ybar, AdjK kb, long N, long nN) {
+                            double *ybar, AdjK kb, const double *__restrict__ hs,
+                            const long long *__restrict__ kp, long N, long nN) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nN) return;
+  const double h = hs[*kp];
   const int f = (int)(i / N);
   const long x = i - (long)f * N;
   double Yb = out_id[i];
@@ -203,7 +248,7 @@ __global__ void k_adj_stage(const double *__restrict__ out_id, const double *__r
   if (lam) Yb += lam[i];
   ybar[i] += Yb;
   for (int j = 0; j < 7; ++j)
-    if (kb.c[j] != 0.0) kb.k[j][i] += kb.c[j] * Yb;
+    if (kb.c[j] != 0.0) kb.k[j][i] += (h * kb.c[j]) * Yb;
 }
 
 }  // namespace rd

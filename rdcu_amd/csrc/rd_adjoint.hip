@@ -297,59 +297,108 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
   }
   HIPCHECK(hipMemsetAsync(lam, 0, nN * sizeof(double), c->stream));
   HIPCHECK(hipMemsetAsync(P, 0, (size_t)std::max(1, nt) * c->rows * sizeof(double), c->stream));
-  o = nout - 1;
-  for (long long k = ns - 1; k >= 0; --k) {
-    const double h = hs[k];
-    const double *y = ych + (size_t)k * nN;
+  // Outputs of each step (contiguous in o) and their dense-output weights
+  // c_oj = hh * sum_p P_jp x^p (dp45.cu:70-76), computed here as before.
+  std::vector<int> obeg(ns, 0), oend(ns, 0);
+  std::vector<double> oc((size_t)7 * nout, 0.0);
+  for (int q = 0; q < nout; ++q) {
+    const long long k = ostep[q];
+    if (oend[k] == 0) obeg[k] = q;
+    oend[k] = q + 1;
+    const double hh = tn[k + 1] - tn[k];
+    const double x = (t_out[q] - tn[k]) / hh;
+    const double xp[4] = {x, x * x, x * x * x, x * x * x * x};
+    for (int j = 0; j < 7; ++j) {
+      double cj = 0.0;
+      for (int pw = 0; pw < 4; ++pw) cj += PP[j][pw] * xp[pw];
+      oc[(size_t)7 * q + j] = hh * cj;
+    }
+  }
+  int *d_obeg, *d_oend;
+  double *d_oc, *ycur;
+  long long *kdev;
+  if ((st = ds.get(&d_obeg, ns)) || (st = ds.get(&d_oend, ns)) ||
+      (st = ds.get(&d_oc, (size_t)7 * nout)) || (st = ds.get(&ycur, nN)) ||
+      (st = ds.get(&kdev, 1)))
+    return st;
+  HIPCHECK(hipMemcpyAsync(d_obeg, obeg.data(), ns * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(d_oend, oend.data(), ns * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(d_oc, oc.data(), oc.size() * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+  // One reverse step (the step index from kdev), with lam_in the adjoint of
+  // the step's end state and ybar_out that of its start.
+  auto enqueue_step = [&](double *lam_in, double *ybar_out) -> int {
+    const unsigned nb = blocks_of(nN);
+    hipLaunchKernelGGL(rd::k_adj_fetch, dim3(nb), dim3(256), 0, c->stream, ych, kdev, ycur, nN);
     // stage inputs from the checkpoint (the forward's operations); the stage
     // values stored by the forward pass, or recomputed
-    if (kst)
-      for (int j = 0; j < 7; ++j) K[j] = kst + ((size_t)k * 7 + j) * nN;
-    else if ((st = enqueue_rhs(c, 0, rd::SRC_PTR, y, rd::DST_PTR, 0, K[0], 0, 0)))
-      return st;
+    int s2;
+    if (!kst && (s2 = enqueue_rhs(c, 0, rd::SRC_PTR, ycur, rd::DST_PTR, 0, K[0], 0, 0))) return s2;
     for (int s = 2; s <= 7; ++s) {
       rd::StageIn si{};
       si.ns = s - 1;
       for (int j = 0; j < s - 1; ++j) {
-        si.k[j] = K[j];
+        si.k[j] = kst ? kst + (size_t)j * nN : K[j];
         si.a[j] = STAGE_A[s - 2][j];
       }
-      hipLaunchKernelGGL(rd::k_stage_in, dim3(blocks_of(nN)), dim3(256), 0, c->stream, y, si, h,
+      hipLaunchKernelGGL(rd::k_stage_in, dim3(nb), dim3(256), 0, c->stream, ycur, si,
+                         (const double *)d_hs, (const long long *)kdev, kst ? 7 * nN : 0L,
                          Ys[s - 1], nN);
-      if (!kst && (st = enqueue_rhs(c, 0, rd::SRC_PTR, Ys[s - 1], rd::DST_PTR, 0, K[s - 1], 0, 0)))
-        return st;
+      if (!kst && (s2 = enqueue_rhs(c, 0, rd::SRC_PTR, Ys[s - 1], rd::DST_PTR, 0, K[s - 1], 0, 0)))
+        return s2;
     }
-    if (!kst) c->nrhs += 7;
-    for (int j = 0; j < 7; ++j)
-      HIPCHECK(hipMemsetAsync(kbar[j], 0, nN * sizeof(double), c->stream));
-    HIPCHECK(hipMemsetAsync(ybar, 0, nN * sizeof(double), c->stream));
-    const double hh = tn[k + 1] - tn[k];
-    for (; o >= 0 && ostep[o] == k; --o) {
-      const double x = (t_out[o] - tn[k]) / hh;
-      const double xp[4] = {x, x * x, x * x * x, x * x * x * x};
-      rd::AdjK kb{};
-      for (int j = 0; j < 7; ++j) {
-        double cj = 0.0;
-        for (int p = 0; p < 4; ++p) cj += PP[j][p] * xp[p];
-        kb.k[j] = kbar[j];
-        kb.c[j] = hh * cj;
-      }
-      hipLaunchKernelGGL(rd::k_adj_dense, dim3(blocks_of(nN)), dim3(256), 0, c->stream, ybar, kb,
-                         gbar + (size_t)o * nN, nN);
-    }
+    rd::AdjK kz{};
+    for (int j = 0; j < 7; ++j) kz.k[j] = kbar[j];
+    hipLaunchKernelGGL(rd::k_adj_zero, dim3(nb), dim3(256), 0, c->stream, kz, ybar_out, nN);
+    hipLaunchKernelGGL(rd::k_adj_dense, dim3(nb), dim3(256), 0, c->stream, ybar_out, kz,
+                       (const double *)gbar, (const int *)d_obeg, (const int *)d_oend,
+                       (const double *)d_oc, (const long long *)kdev, nN);
     for (int s = 7; s >= 1; --s) {
-      const double *u = s == 1 ? y : Ys[s - 1];
-      if ((st = vjp(c, u, kbar[s - 1], out_id, Gk, Dk, P, tmap))) return st;
+      const double *u = s == 1 ? ycur : Ys[s - 1];
+      if ((s2 = vjp(c, u, kbar[s - 1], out_id, Gk, Dk, P, tmap))) return s2;
       rd::AdjK kb{};
       for (int j = 0; j < 7; ++j) kb.k[j] = kbar[j];
       if (s >= 2)
-        for (int j = 0; j < s - 1; ++j) kb.c[j] = h * STAGE_A[s - 2][j];
-      hipLaunchKernelGGL(rd::k_adj_stage, dim3(blocks_of(nN)), dim3(256), 0, c->stream, out_id, Dk,
-                         fmap, nf, s == 7 ? lam : nullptr, ybar, kb, N, nN);
+        for (int j = 0; j < s - 1; ++j) kb.c[j] = STAGE_A[s - 2][j];
+      hipLaunchKernelGGL(rd::k_adj_stage, dim3(nb), dim3(256), 0, c->stream,
+                         (const double *)out_id, (const double *)Dk, (const int *)fmap, nf,
+                         (const double *)(s == 7 ? lam_in : nullptr), ybar_out, kb,
+                         (const double *)d_hs, (const long long *)kdev, N, nN);
     }
+    hipLaunchKernelGGL(rd::k_adj_next, dim3(1), dim3(64), 0, c->stream, kdev);
     HIPCHECK(hipGetLastError());
-    std::swap(lam, ybar);
+    return RDCU_OK;
+  };
+  // The step is captured twice, once per role of the two adjoint buffers
+  // (they swap every step), and replayed ns times.
+  struct Execs {
+    hipStream_t s;
+    hipGraphExec_t x[2] = {nullptr, nullptr};
+    ~Execs() {
+      (void)hipStreamSynchronize(s);
+      for (auto &e : x)
+        if (e) (void)hipGraphExecDestroy(e);
+    }
+  } ex{c->stream};
+  double *buf[2] = {lam, ybar};
+  for (int par = 0; par < 2 && par < ns; ++par) {
+    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int s2 = enqueue_step(buf[par], buf[1 - par]);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+    if (s2 || ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return s2 ? s2 : fail(RDCU_EDEVICE, std::string("reverse step capture: ") + hipGetErrorString(ce));
+    }
+    const hipError_t ie = hipGraphInstantiate(&ex.x[par], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess)
+      return fail(RDCU_EDEVICE, std::string("reverse step graph: ") + hipGetErrorString(ie));
   }
+  hipLaunchKernelGGL(rd::k_adj_setk, dim3(1), dim3(64), 0, c->stream, kdev, ns - 1);
+  for (long long i = 0; i < ns; ++i) HIPCHECK(hipGraphLaunch(ex.x[i & 1], c->stream));
+  if (!kst) c->nrhs += 7 * ns;
+  lam = buf[ns & 1];
   std::vector<double> hp((size_t)std::max(1, nt) * c->rows);
   HIPCHECK(hipMemcpyAsync(hp.data(), P, hp.size() * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
