@@ -411,11 +411,30 @@ __device__ __forceinline__ void c2r_pre(const Grid &g, const FacList &fl, const 
   }
 }
 
-// LDS of one row of the x-inverse pass (complex units): G FFT lines,
-// (n+nf)*Nx values, the dictionary program.  The twiddles are read from
-// global memory (cached), which leaves more rows per CU.
+// LDS of one row of the x-inverse pass (complex units).  The value rows
+// (n identity rows, then nf factor rows, Nx doubles each) come first.  The
+// FFT lines start where the rows of the last factor group begin: those rows
+// are written only after the group's transform, when the lines are free.
+// After them come the n stage-result (or error-square) rows, then the
+// dictionary program.  The twiddles are read from global memory (cached).
+struct RowiLayout {
+  long early;  // complex offset of the FFT lines: rows written before the last transform
+  long late;   // complex size of the rows written after it (inside the lines' span)
+  long tail;   // complex size from the lines' start to the program
+};
+__host__ __device__ inline RowiLayout rowi_layout(int L, int Nx, int n, int nf, int G) {
+  const int ng = nf > 0 ? (nf + G - 1) / G : 0;
+  const int nearly = n + (ng > 1 ? (ng - 1) * G : 0);
+  RowiLayout o;
+  o.early = (long)nearly * Nx / 2;  // Nx is even
+  o.late = (long)(n + nf - nearly) * Nx / 2;
+  const long lines = (long)G * padded_len(L), rest = o.late + (long)n * Nx / 2;
+  o.tail = lines > rest ? lines : rest;
+  return o;
+}
 __host__ __device__ inline long rowi_region(int L, int Nx, int n, int nf, int nwords, int G) {
-  return (long)G * padded_len(L) + ((long)(n + nf) * Nx + 1) / 2 + (nwords + 3) / 4;
+  const RowiLayout o = rowi_layout(L, Nx, n, nf, G);
+  return o.early + o.tail + (nwords + 3) / 4;
 }
 
 // One grid row r, by the nt threads tid = 0..nt-1 of a row group whose LDS
@@ -436,13 +455,15 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   const int nf = GK::nf >= 0 ? GK::nf : fl.nf;
   const int PL = padded_len(L);
   const cx *tw = px.tw;
-  cx *buf = smem;
-  double *vals = reinterpret_cast<double *>(buf + G * PL);
-  // the stage results for the fused next stage share the FFT lines' LDS
-  // (free between the factor transforms and the next stage's transform; the
-  // error pass, which uses it for the squares, has no next stage)
-  double *fv = reinterpret_cast<double *>(buf);
-  int *prog = reinterpret_cast<int *>(vals + (long)(n + nf) * Nx);
+  const RowiLayout lay = rowi_layout(L, Nx, n, nf, G);
+  double *vals = reinterpret_cast<double *>(smem);
+  cx *buf = smem + lay.early;
+  // the stage results for the fused next stage follow the value rows, in the
+  // FFT lines' span (free between the factor transforms and the next stage's
+  // transform; the error pass, which uses them for the squares, has no next
+  // stage)
+  double *fv = reinterpret_cast<double *>(buf + lay.late);
+  int *prog = reinterpret_cast<int *>(buf + lay.tail);
   cx *lb = buf + gl * PL;
   const double *src =
       src_kind == SRC_Y ? b.y[st->iy] : (src_kind == SRC_YT ? b.y[st->iyt] : src_ptr);
@@ -501,7 +522,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   else if (dst_kind == DST_K1) out = b.k[st->k1s];
   else if (dst_kind == DST_K7) out = b.k[st->k7s];
   else out = b.k[dst_slot];
-  double *sq = reinterpret_cast<double *>(buf);  // G*L complex >= n*Nx doubles
+  double *sq = fv;  // n*Nx doubles
   const double *yv = nullptr, *k1 = nullptr;
   double h = 0, atol = 0, rtol = 0;
   if (err) {
