@@ -353,7 +353,7 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
 }
 
 template <int E, bool NT, int LS = 0>
-__global__ __launch_bounds__(256) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
+__global__ __launch_bounds__(512) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
                                                  int check_active, Bufs b, FacList fl, int mode,
                                                  int fpw) {
   if (check_active && !attempt_active(st)) return;

@@ -732,7 +732,7 @@ int fixed_shapes(rdcu_ctx *c, const char *spec) {
   const bool ok =
       rpb >= 1 && rpb * T <= 256 &&
       (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx) <= 160 * 1024 &&
-      (c->ndim < 2 || (tc >= 1 && tc * (Lmax / Emin) <= 256 &&
+      (c->ndim < 2 || (tc >= 1 && tc * (Lmax / Emin) <= 512 &&
                        (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) <=
                            160 * 1024 &&
                        fpw >= 1 && fpw <= std::max(1, c->max_fpf))) &&
@@ -800,7 +800,7 @@ int autotune(rdcu_ctx *c) {
         Tmax = std::max(Tmax, c->Ns[1] / c->E1);
       }
       for (int tc = std::max(1, 64 / Tmax); tc <= 64; tc *= 2) {
-        if (tc * Tmax > 256 ||
+        if (tc * Tmax > 512 ||
             (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) > 160 * 1024)
           break;
         for (int fpw = c->max_fpf; fpw >= 1; fpw = fpw > 1 ? (fpw + 1) / 2 : 0) {
