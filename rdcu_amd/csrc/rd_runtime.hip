@@ -713,6 +713,12 @@ void build_jit(rdcu_ctx *c) {
 // Shapes given by the caller instead of the autotuner (profiling runs that
 // must see only the launches of one shape), checked against the same limits.
 
+int pow2_above(int x) {
+  int p = 1;
+  while (p <= x) p *= 2;
+  return p;
+}
+
 int fixed_shapes(rdcu_ctx *c, const char *spec) {
   int rpb = 0, tc = 0, fpw = 0, G = 0, R = 1, Ec = 0;
   const int got = sscanf(spec, "%d,%d,%d,%d,%d,%d", &rpb, &tc, &fpw, &G, &R, &Ec);
@@ -736,7 +742,7 @@ int fixed_shapes(rdcu_ctx *c, const char *spec) {
                        (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) <=
                            160 * 1024 &&
                        fpw >= 1 && fpw <= std::max(1, c->max_fpf))) &&
-      G >= c->n && G <= std::max(c->n, c->nf_used) && R >= 1 && (R == 1 || c->Ex <= 8) &&
+      G >= c->n && R >= 1 && (R == 1 || c->Ex <= 8) &&
       R * G * T <= 256 &&
       c->rows % R == 0 && rowi_lds(c, G, R) <= 160 * 1024;
   if (!ok) return fail(RDCU_EINVAL, std::string("RDCU_SHAPES out of range: ") + spec);
@@ -824,10 +830,13 @@ int autotune(rdcu_ctx *c) {
     c->E1 = bestE1;
     c->cr_fpw = best_fpw;
   }
-  // x inverse: factor lines per group, rows per block (dividing the rows)
+  // x inverse: factor lines per group, rows per block (dividing the rows).
+  // Groups wider than the factor count (powers of two above it) leave lines
+  // idle but spread the dictionary and the stage sums over more threads,
+  // which pays when there are few rows (1-D: one row).
   best = 1e30;
-  for (int G = c->n; G <= std::max(c->n, c->nf_used); ++G) {
-    if (G * T > 256) break;
+  const int Gf = std::max(c->n, c->nf_used);
+  for (int G = c->n; G * T <= 256; G = G < Gf ? G + 1 : (G == Gf ? pow2_above(Gf) : 2 * G)) {
     for (int R = 1; R * G * T <= 256 && (R == 1 || c->Ex <= 8); R *= 2) {
       if (c->rows % R) break;
       if (rowi_lds(c, G, R) > 160 * 1024) break;
