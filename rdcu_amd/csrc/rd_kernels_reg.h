@@ -699,6 +699,38 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
   }
 }
 
+// One whole DP45 attempt of a 1-D problem (a single grid row) in one
+// workgroup: the stage-2 x forward pass, the six fused x-inverse passes and
+// the step control run as phases separated by barriers instead of as eight
+// kernels.  Each phase is the body the separate kernel runs, so the values
+// are the same bits.  Block: G*T threads (G = blockDim/T line groups, G >= n);
+// LDS: the largest phase (x forward: L + G*padded_len(L) complex; x inverse:
+// rowi_region; control: n doubles).
+struct AttemptCoef {
+  StageCoef sc2;    // stage-2 sum, formed by the x forward phase
+  StageCoef nx[6];  // next-stage sums fused after stages 2..7 (nx[5]: none)
+  ErrCoef ec;
+};
+template <int E, class DE, int LS = 0, class GK = RunGeom>
+__device__ __forceinline__ void attempt1d_body(const Grid &g, const FftPlan &px,
+                                               const cx *__restrict__ twN, DevState *st,
+                                               const Bufs &b, const FacList &fl, const DictP &dict,
+                                               const AttemptCoef &ac, int fixed) {
+  extern __shared__ cx smem[];
+  if (!attempt_active(st)) return;
+  const int L = LS > 0 ? LS : g.L, T = L / E;
+  const int nt = blockDim.x, G = nt / T;
+  rowf_body<E, false, LS>(0, g, px, twN, st, b, SRC_STAGE, nullptr, ac.sc2, G);
+  __syncthreads();
+  for (int s = 2; s <= 7; ++s) {
+    rowi_body<E, DE, LS, GK>(0, g, px, twN, st, b, fl, dict, SRC_YT, nullptr,
+                             s < 7 ? DST_KS : DST_K7, s - 1, nullptr, s == 7 && !fixed, ac.ec,
+                             ac.nx[s - 2], G, smem, threadIdx.x, nt);
+    __syncthreads();
+  }
+  control_step(st, b.part, (long)g.n * g.rows, (long)g.n * g.N, reinterpret_cast<double *>(smem));
+}
+
 template <int E>
 __global__ __launch_bounds__(256) void k_rowi_reg(Grid g, FftPlan px, const cx *__restrict__ twN,
                                                   DevState *st, int check_active, Bufs b,

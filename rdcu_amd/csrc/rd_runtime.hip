@@ -131,6 +131,9 @@ struct rdcu_ctx {
   // x-inverse kernel with the dictionary compiled in (hiprtc); null if not built
   hipModule_t jit_mod = nullptr;
   hipFunction_t jit_rowi = nullptr;
+  // 1-D: a whole attempt in one workgroup (null if not built or not eligible)
+  hipFunction_t jit_att = nullptr;
+  size_t lds_att = 0;
   std::string jit_log;
   std::vector<int> h_tfield, h_tstart, h_fslot, h_fpow;
   std::vector<double> h_tcoef;
@@ -669,6 +672,22 @@ bool jit_enabled() {
 // Compiles the x-inverse kernel with the dictionary structure built in.  On
 // any failure the interpreter kernel (k_rowi_reg) stays in use; RDCU_JIT=0
 // disables it.
+// LDS of the single-workgroup 1-D attempt (256 threads: G = 256/T line
+// groups), or 0 where it does not apply (not 1-D, 24 values per thread, fewer
+// line groups than fields, or more LDS than a CU has).
+size_t attempt1d_lds(const rdcu_ctx *c) {
+  if (c->ndim != 1 || !c->reg || c->Ex != 8) return 0;
+  const int T = c->L / c->Ex;
+  if (T < 1 || 256 % T) return 0;
+  const int G = 256 / T;
+  if (G < c->n) return 0;
+  size_t lds = (size_t)(c->L + (long)G * rd::padded_len(c->L)) * sizeof(cx);
+  lds = std::max(lds, (size_t)rd::rowi_region(c->L, c->Nx, c->n, c->nf_used, c->dictp.nwords, G) *
+                          sizeof(cx));
+  lds = std::max(lds, (size_t)c->n * sizeof(double));
+  return lds <= 160 * 1024 ? lds : 0;
+}
+
 void build_jit(rdcu_ctx *c) {
   if (!jit_enabled()) return;
   std::string src = "#include \"rd_kernels_reg.h\"\n" +
@@ -693,6 +712,13 @@ void build_jit(rdcu_ctx *c) {
   src += "  rd::rowi_rows<" + E + ", rd::JitDict, " + std::to_string(c->L) +
          ", rd::JitGeom>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot, "
          "dst_ptr, err, ec, nx, G);\n}\n";
+  const size_t att_lds = attempt1d_lds(c);
+  if (att_lds)
+    src += "extern \"C\" __global__ __launch_bounds__(256) void rdcu_attempt1d_jit(rd::Grid g, "
+           "rd::FftPlan px, const rd::cx *__restrict__ twN, rd::DevState *st, rd::Bufs b, "
+           "rd::FacList fl, rd::DictP dict, rd::AttemptCoef ac, int fixed) {\n"
+           "  rd::attempt1d_body<" + E + ", rd::JitDict, " + std::to_string(c->L) +
+           ", rd::JitGeom>(g, px, twN, st, b, fl, dict, ac, fixed);\n}\n";
   std::vector<char> code;
   if (!jit_compile(src, jit_arch(c->dev), code, c->jit_log)) return;
   if (hipModuleLoadData(&c->jit_mod, code.data()) != hipSuccess ||
@@ -703,6 +729,10 @@ void build_jit(rdcu_ctx *c) {
     c->jit_rowi = nullptr;
     return;
   }
+  if (att_lds && hipModuleGetFunction(&c->jit_att, c->jit_mod, "rdcu_attempt1d_jit") == hipSuccess)
+    c->lds_att = att_lds;
+  else
+    c->jit_att = nullptr;
   c->jit_log = "ok";
 }
 
@@ -931,6 +961,24 @@ int enqueue_all_factors(rdcu_ctx *c) {
 std::vector<Launch> attempt_launches_reg(rdcu_ctx *c);
 
 int enqueue_attempt(rdcu_ctx *c) {
+  if (c->reg && c->jit_att) {  // 1-D: the whole attempt, control included, in one workgroup
+    rd::AttemptCoef ac{};
+    ac.sc2 = stage_coef(2);
+    for (int s = 2; s <= 7; ++s) ac.nx[s - 2] = s < 7 ? stage_coef(s + 1) : rd::StageCoef{};
+    for (int j = 0; j < 7; ++j) ac.ec.e[j] = EE[j];
+    rd::Grid a_g = c->grid;
+    rd::FftPlan a_px = c->px;
+    const rd::cx *a_tw = c->twN;
+    rd::DevState *a_st = c->st;
+    rd::Bufs a_b = c->bufs;
+    rd::FacList a_fl = c->fl_used;
+    rd::DictP a_d = c->dictp;
+    int a_fixed = c->fixed;
+    void *args[] = {&a_g, &a_px, &a_tw, &a_st, &a_b, &a_fl, &a_d, &ac, &a_fixed};
+    HIPCHECK(hipModuleLaunchKernel(c->jit_att, 1, 1, 1, 256, 1, 1, (unsigned)c->lds_att,
+                                   c->stream, args, nullptr));
+    return RDCU_OK;
+  }
   if (c->reg) {
     for (auto &l : attempt_launches_reg(c)) l.go();
   }
