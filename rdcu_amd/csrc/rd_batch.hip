@@ -284,7 +284,7 @@ int autotune_b(rdcu_batch *b, BatchClass &cl) {
   best = 1e30;
   for (int G = c->n; G <= std::max(c->n, cl.nf_max); ++G) {
     if (G * T > 256) break;
-    for (int R = 1; R * G * T <= 256 && c->rows % R == 0 && (R == 1 || cl.Ex <= 8); R *= 2) {
+    for (int R = 1; R * G * T <= 256 && c->rows % R == 0; R *= 2) {
       if (rowi_lds(cl, G, R) > 160 * 1024) break;
       double tm = time_of([&]() { rowi_b(b, cl, rd::SRC_Y, rd::DST_KS, 1, 0, none, G, R, 0); });
       snprintf(line, sizeof line, "row_inv G=%d R=%d %.1fus; ", G, R, tm * 1e3);

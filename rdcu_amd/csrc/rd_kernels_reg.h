@@ -674,8 +674,9 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
 }
 
 // Block = R rows x G*T threads (R = blockDim / (G*T)), row r = blockIdx*R +
-// sub.  With 24 values per thread the per-row thread range and LDS offset
-// cost registers the kernel does not have, so those kernels take R = 1.
+// sub.  (With 24 values per thread the kernels sit at 512 registers and
+// spill a few values for the per-row offsets; blocks of a single row would
+// leave most of each wave empty at short rows, which costs far more.)
 template <int E, class DE, int LS = 0, class GK = RunGeom>
 __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
@@ -685,18 +686,13 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
                                           const StageCoef &nx, int G) {
   extern __shared__ cx smem[];
   const int L = LS > 0 ? LS : g.L;
-  if constexpr (E <= 8) {
-    const int Nx = GK::Nx ? GK::Nx : g.Nx, n = GK::n ? GK::n : g.n;
-    const int nf = GK::nf >= 0 ? GK::nf : fl.nf;
-    const long region = rowi_region(L, Nx, n, nf, dict.nwords, G);
-    const int gt = G * (L / E), sub = threadIdx.x / gt;
-    rowi_body<E, DE, LS, GK>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl,
-                             dict, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
-                             smem + sub * region, threadIdx.x - sub * gt, gt);
-  } else {
-    rowi_body<E, DE, LS, GK>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
-                             dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x);
-  }
+  const int Nx = GK::Nx ? GK::Nx : g.Nx, n = GK::n ? GK::n : g.n;
+  const int nf = GK::nf >= 0 ? GK::nf : fl.nf;
+  const long region = rowi_region(L, Nx, n, nf, dict.nwords, G);
+  const int gt = G * (L / E), sub = threadIdx.x / gt;
+  rowi_body<E, DE, LS, GK>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl,
+                           dict, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
+                           smem + sub * region, threadIdx.x - sub * gt, gt);
 }
 
 // One whole DP45 attempt of a 1-D problem (a single grid row) in one
@@ -793,28 +789,16 @@ __device__ __forceinline__ void rowi_batch(const Inst *__restrict__ in, int nins
                                            int check, long region) {
   extern __shared__ cx smem[];
   // row regions are sized for the largest factor count and program of the class
-  if constexpr (E <= 8) {
-    const int gt = G * ((LS > 0 ? LS : px.L) / E), R = blockDim.x / gt, sub = threadIdx.x / gt;
-    const long it = (long)blockIdx.x * R + sub;  // R divides rows: one instance per block
-    if (it >= (long)ninst * rows) return;
-    const int i = (int)(it / rows);
-    DevState *st = in[i].st;
-    if (check && !attempt_active(st)) return;
-    const Inst &I = in[i];
-    rowi_body<E, DE, LS, GK>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind,
-                             nullptr, dst_kind, dst_slot, nullptr, err, ec, nx, G,
-                             smem + sub * region, threadIdx.x - sub * gt, gt);
-  } else {  // one row per block (see rowi_rows)
-    const long it = blockIdx.x;
-    if (it >= (long)ninst * rows) return;
-    const int i = (int)(it / rows);
-    DevState *st = in[i].st;
-    if (check && !attempt_active(st)) return;
-    const Inst &I = in[i];
-    rowi_body<E, DE, LS, GK>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind,
-                             nullptr, dst_kind, dst_slot, nullptr, err, ec, nx, G, smem,
-                             threadIdx.x, blockDim.x);
-  }
+  const int gt = G * ((LS > 0 ? LS : px.L) / E), R = blockDim.x / gt, sub = threadIdx.x / gt;
+  const long it = (long)blockIdx.x * R + sub;  // R divides rows: one instance per block
+  if (it >= (long)ninst * rows) return;
+  const int i = (int)(it / rows);
+  DevState *st = in[i].st;
+  if (check && !attempt_active(st)) return;
+  const Inst &I = in[i];
+  rowi_body<E, DE, LS, GK>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind,
+                           nullptr, dst_kind, dst_slot, nullptr, err, ec, nx, G,
+                           smem + sub * region, threadIdx.x - sub * gt, gt);
 }
 
 template <int E>

@@ -772,7 +772,7 @@ int fixed_shapes(rdcu_ctx *c, const char *spec) {
                        (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) <=
                            160 * 1024 &&
                        fpw >= 1 && fpw <= std::max(1, c->max_fpf))) &&
-      G >= c->n && R >= 1 && (R == 1 || c->Ex <= 8) &&
+      G >= c->n && R >= 1 &&
       R * G * T <= 256 &&
       c->rows % R == 0 && rowi_lds(c, G, R) <= 160 * 1024;
   if (!ok) return fail(RDCU_EINVAL, std::string("RDCU_SHAPES out of range: ") + spec);
@@ -867,7 +867,7 @@ int autotune(rdcu_ctx *c) {
   best = 1e30;
   const int Gf = std::max(c->n, c->nf_used);
   for (int G = c->n; G * T <= 256; G = G < Gf ? G + 1 : (G == Gf ? pow2_above(Gf) : 2 * G)) {
-    for (int R = 1; R * G * T <= 256 && (R == 1 || c->Ex <= 8); R *= 2) {
+    for (int R = 1; R * G * T <= 256; R *= 2) {
       if (c->rows % R) break;
       if (rowi_lds(c, G, R) > 160 * 1024) break;
       double tm = time_of([&]() {
