@@ -18,10 +18,22 @@
 
 namespace rd {
 
-// Block = G lines of T = L/E threads; one grid row r per block.  LDS: tw (L)
-// and G*padded_len(L) complex, then (n+nf)*Nx values, (n+nf)*Nx weights,
-// n*Nx adjoint inputs, blockDim reduction doubles, the dictionary program.
-// P[tmap[p]*rows + r] += row sum of w*term for the program's p-th term.
+// Sum over the 64 lanes of a wave (butterfly: every lane gets the sum).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// Block = G lines of T = L/E threads; one grid row r per block.  LDS:
+// G*padded_len(L) complex, then (n+nf)*Nx values, (n+nf)*Nx weights, n*Nx
+// adjoint inputs, the dictionary program (padded to 8 bytes), then the
+// reduction doubles (max(blockDim, terms * blockDim/64)).  The twiddles are
+// read from global memory.
+// P[tmap[p]*rows + r] += row sum of w*term for the program's p-th term: with
+// whole waves, each term's partial sums are reduced within the wave and the
+// waves' sums added at the end (no barrier per term); otherwise by a tree in
+// LDS per term.
 template <int E>
 __global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *__restrict__ twN,
                                                   Bufs b, FacList fl, DictP dict,
@@ -37,13 +49,15 @@ __global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *
   const int gl = tid / T, t = tid - gl * T;
   const int nf = fl.nf, ns = n + nf;
   const int PL = padded_len(L);
-  cx *tw = smem, *buf = smem + L;
+  const cx *tw = px.tw;
+  cx *buf = smem;
   double *vals = reinterpret_cast<double *>(buf + G * PL);
   double *gacc = vals + (long)ns * Nx;
   double *wrow = gacc + (long)ns * Nx;
-  double *red = wrow + (long)n * Nx;
-  int *prog = reinterpret_cast<int *>(red + nt);
-  for (int i = tid; i < L; i += nt) tw[i] = px.tw[i];
+  int *prog = reinterpret_cast<int *>(wrow + (long)n * Nx);
+  double *red = reinterpret_cast<double *>(prog + ((dict.nwords + 1) & ~1));
+  const bool waves = (nt & 63) == 0;
+  const int nwv = nt >> 6;
   for (int i = tid; i < dict.nwords; i += nt) prog[i] = dict.prog[i];
   for (int w = tid; w < n * Nx; w += nt) {
     const long gi = (long)(w / Nx) * N + r * Nx + (w % Nx);
@@ -100,6 +114,11 @@ __global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *
           gacc[fp[2 * j] * Nx + x] += wv * cc * d;
         }
       }
+      if (waves) {
+        acc = wave_sum(acc);
+        if ((tid & 63) == 0) red[pt * nwv + (tid >> 6)] = acc;
+        continue;
+      }
       red[tid] = acc;
       __syncthreads();
       int p2 = 1;
@@ -110,6 +129,16 @@ __global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *
       }
       if (tid == 0) P[(long)tmap[pt] * rows + r] += red[0];
       __syncthreads();
+    }
+  }
+  if (waves) {
+    int npt = 0;  // terms of all fields
+    for (int q = prog[0]; q < prog[n]; q += 3 + 2 * prog[q]) ++npt;
+    __syncthreads();
+    for (int q = tid; q < npt; q += nt) {
+      double sacc = red[q * nwv];
+      for (int w = 1; w < nwv; ++w) sacc += red[q * nwv + w];
+      P[(long)tmap[q] * rows + r] += sacc;
     }
   }
   for (int w = tid; w < n * Nx; w += nt)

@@ -109,11 +109,15 @@ int vjp(rdcu_ctx *c, const double *u, const double *w, double *out_id, double *G
   rowf(c, 0, rd::SRC_PTR, u, none, c->rf_rpb);
   for (auto &l : col_launches_reg(c, 0, c->cr_tc)) l.go();
   const int nf = c->nf_used, T = c->L / c->Ex;
-  const int G = std::max(1, std::min(nf, 256 / T));
+  // all 256 threads when the line groups divide them (idle lines, but the
+  // per-term loop over the row's points gets the most threads)
+  const int G = (256 % T == 0 && 256 / T >= nf) ? 256 / T : std::max(1, std::min(nf, 256 / T));
   const int nt = G * T;
-  const size_t lds = (size_t)(c->L + G * rd::padded_len(c->L)) * sizeof(cx) +
+  const int nterms = (int)c->h_tcoef.size();
+  const size_t nred = std::max<size_t>(nt, (nt % 64 == 0) ? (size_t)nterms * (nt / 64) : 0);
+  const size_t lds = (size_t)G * rd::padded_len(c->L) * sizeof(cx) +
                      (size_t)(2 * (c->n + nf) + c->n) * c->Nx * sizeof(double) +
-                     (size_t)nt * sizeof(double) + (size_t)c->dictp.nwords * sizeof(int);
+                     (size_t)((c->dictp.nwords + 1) & ~1) * sizeof(int) + nred * sizeof(double);
   if (lds > 160 * 1024) return fail(RDCU_EUNSUPPORTED, "row too long for the adjoint kernel");
   if (c->Ex == 8)
     hipLaunchKernelGGL(rd::k_rowi_vjp<8>, dim3((unsigned)c->rows), dim3(nt), lds, c->stream,
