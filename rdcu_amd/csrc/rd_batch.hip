@@ -282,9 +282,12 @@ int autotune_b(rdcu_batch *b, BatchClass &cl) {
     }
   }
   best = 1e30;
-  for (int G = c->n; G <= std::max(c->n, cl.nf_max); ++G) {
-    if (G * T > 256) break;
-    for (int R = 1; R * G * T <= 256 && c->rows % R == 0; R *= 2) {
+  // factor lines per group, also wider than the factor count (powers of two;
+  // idle lines, but more threads per row: with 24 values per thread a block
+  // holds one row, and at short rows G*T is a fraction of a wave)
+  const int Gf = std::max(c->n, cl.nf_max);
+  for (int G = c->n; G * T <= 256; G = G < Gf ? G + 1 : (G == Gf ? pow2_above(Gf) : 2 * G)) {
+    for (int R = 1; R * G * T <= 256 && c->rows % R == 0 && (R == 1 || cl.Ex <= 8); R *= 2) {
       if (rowi_lds(cl, G, R) > 160 * 1024) break;
       double tm = time_of([&]() { rowi_b(b, cl, rd::SRC_Y, rd::DST_KS, 1, 0, none, G, R, 0); });
       snprintf(line, sizeof line, "row_inv G=%d R=%d %.1fus; ", G, R, tm * 1e3);
@@ -351,7 +354,8 @@ void build_jit_b(rdcu_batch *b) {
     for (size_t gi = 0; gi < cl.groups.size(); ++gi) {
       const std::string nm = "rdcu_rowi_b_" + std::to_string(ci) + "_" + std::to_string(gi);
       hipFunction_t f = nullptr;
-      if (hipModuleGetFunction(&f, mod, nm.c_str()) == hipSuccess) cl.groups[gi].fn = f;
+      if (hipModuleGetFunction(&f, mod, nm.c_str()) == hipSuccess && !jit_spills(f))
+        cl.groups[gi].fn = f;
     }
   }
   b->jit_log = "ok";

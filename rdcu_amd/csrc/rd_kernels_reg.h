@@ -352,8 +352,14 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   }
 }
 
+// Up to 512 threads per block with 8 values per thread (204 registers fit
+// two waves per SIMD); 16 and 24 values per thread need more than 256
+// registers, so their blocks stay at 256 threads and nothing spills.
+template <int E>
+constexpr int col_max_threads() { return E == 8 ? 512 : 256; }
+
 template <int E, bool NT, int LS = 0>
-__global__ __launch_bounds__(512) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
+__global__ __launch_bounds__(col_max_threads<E>()) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
                                                  int check_active, Bufs b, FacList fl, int mode,
                                                  int fpw) {
   if (check_active && !attempt_active(st)) return;
@@ -674,9 +680,8 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
 }
 
 // Block = R rows x G*T threads (R = blockDim / (G*T)), row r = blockIdx*R +
-// sub.  (With 24 values per thread the kernels sit at 512 registers and
-// spill a few values for the per-row offsets; blocks of a single row would
-// leave most of each wave empty at short rows, which costs far more.)
+// sub.  With 24 values per thread the per-row thread range and LDS offset
+// cost registers the kernel does not have, so those kernels take R = 1.
 template <int E, class DE, int LS = 0, class GK = RunGeom>
 __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
@@ -686,13 +691,18 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
                                           const StageCoef &nx, int G) {
   extern __shared__ cx smem[];
   const int L = LS > 0 ? LS : g.L;
-  const int Nx = GK::Nx ? GK::Nx : g.Nx, n = GK::n ? GK::n : g.n;
-  const int nf = GK::nf >= 0 ? GK::nf : fl.nf;
-  const long region = rowi_region(L, Nx, n, nf, dict.nwords, G);
-  const int gt = G * (L / E), sub = threadIdx.x / gt;
-  rowi_body<E, DE, LS, GK>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl,
-                           dict, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
-                           smem + sub * region, threadIdx.x - sub * gt, gt);
+  if constexpr (E <= 8) {
+    const int Nx = GK::Nx ? GK::Nx : g.Nx, n = GK::n ? GK::n : g.n;
+    const int nf = GK::nf >= 0 ? GK::nf : fl.nf;
+    const long region = rowi_region(L, Nx, n, nf, dict.nwords, G);
+    const int gt = G * (L / E), sub = threadIdx.x / gt;
+    rowi_body<E, DE, LS, GK>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl,
+                             dict, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
+                             smem + sub * region, threadIdx.x - sub * gt, gt);
+  } else {
+    rowi_body<E, DE, LS, GK>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
+                             dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x);
+  }
 }
 
 // One whole DP45 attempt of a 1-D problem (a single grid row) in one
@@ -789,16 +799,28 @@ __device__ __forceinline__ void rowi_batch(const Inst *__restrict__ in, int nins
                                            int check, long region) {
   extern __shared__ cx smem[];
   // row regions are sized for the largest factor count and program of the class
-  const int gt = G * ((LS > 0 ? LS : px.L) / E), R = blockDim.x / gt, sub = threadIdx.x / gt;
-  const long it = (long)blockIdx.x * R + sub;  // R divides rows: one instance per block
-  if (it >= (long)ninst * rows) return;
-  const int i = (int)(it / rows);
-  DevState *st = in[i].st;
-  if (check && !attempt_active(st)) return;
-  const Inst &I = in[i];
-  rowi_body<E, DE, LS, GK>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind,
-                           nullptr, dst_kind, dst_slot, nullptr, err, ec, nx, G,
-                           smem + sub * region, threadIdx.x - sub * gt, gt);
+  if constexpr (E <= 8) {
+    const int gt = G * ((LS > 0 ? LS : px.L) / E), R = blockDim.x / gt, sub = threadIdx.x / gt;
+    const long it = (long)blockIdx.x * R + sub;  // R divides rows: one instance per block
+    if (it >= (long)ninst * rows) return;
+    const int i = (int)(it / rows);
+    DevState *st = in[i].st;
+    if (check && !attempt_active(st)) return;
+    const Inst &I = in[i];
+    rowi_body<E, DE, LS, GK>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind,
+                             nullptr, dst_kind, dst_slot, nullptr, err, ec, nx, G,
+                             smem + sub * region, threadIdx.x - sub * gt, gt);
+  } else {  // one row per block (see rowi_rows)
+    const long it = blockIdx.x;
+    if (it >= (long)ninst * rows) return;
+    const int i = (int)(it / rows);
+    DevState *st = in[i].st;
+    if (check && !attempt_active(st)) return;
+    const Inst &I = in[i];
+    rowi_body<E, DE, LS, GK>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind,
+                             nullptr, dst_kind, dst_slot, nullptr, err, ec, nx, G, smem,
+                             threadIdx.x, blockDim.x);
+  }
 }
 
 template <int E>

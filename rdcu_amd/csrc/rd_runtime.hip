@@ -672,6 +672,18 @@ bool jit_enabled() {
 // Compiles the x-inverse kernel with the dictionary structure built in.  On
 // any failure the interpreter kernel (k_rowi_reg) stays in use; RDCU_JIT=0
 // disables it.
+// True if a generated kernel keeps values in scratch (private) memory.  Such
+// kernels are not used: every kernel here is meant to run from registers and
+// LDS, and a spilling strided kernel once gave wrong values on the GPU.
+bool jit_spills(hipFunction_t f) {
+  int local = 0;
+  if (hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f) != hipSuccess) {
+    (void)hipGetLastError();
+    return true;
+  }
+  return local > 0;
+}
+
 // LDS of the single-workgroup 1-D attempt (256 threads: G = 256/T line
 // groups), or 0 where it does not apply (not 1-D, 24 values per thread, fewer
 // line groups than fields, or more LDS than a CU has).
@@ -722,14 +734,16 @@ void build_jit(rdcu_ctx *c) {
   std::vector<char> code;
   if (!jit_compile(src, jit_arch(c->dev), code, c->jit_log)) return;
   if (hipModuleLoadData(&c->jit_mod, code.data()) != hipSuccess ||
-      hipModuleGetFunction(&c->jit_rowi, c->jit_mod, "rdcu_rowi_jit") != hipSuccess) {
-    c->jit_log = "module load failed";
+      hipModuleGetFunction(&c->jit_rowi, c->jit_mod, "rdcu_rowi_jit") != hipSuccess ||
+      jit_spills(c->jit_rowi)) {
+    c->jit_log = c->jit_rowi && jit_spills(c->jit_rowi) ? "spills to scratch" : "module load failed";
     if (c->jit_mod) (void)hipModuleUnload(c->jit_mod);
     c->jit_mod = nullptr;
     c->jit_rowi = nullptr;
     return;
   }
-  if (att_lds && hipModuleGetFunction(&c->jit_att, c->jit_mod, "rdcu_attempt1d_jit") == hipSuccess)
+  if (att_lds && hipModuleGetFunction(&c->jit_att, c->jit_mod, "rdcu_attempt1d_jit") == hipSuccess &&
+      !jit_spills(c->jit_att))
     c->lds_att = att_lds;
   else
     c->jit_att = nullptr;
@@ -742,6 +756,13 @@ void build_jit(rdcu_ctx *c) {
 // current state.
 // Shapes given by the caller instead of the autotuner (profiling runs that
 // must see only the launches of one shape), checked against the same limits.
+
+// Largest strided-pass block for the values per thread in use (the
+// kernels' launch bounds).
+int col_threads(const rdcu_ctx *c) {
+  const bool e8 = c->E0 == 8 && (c->ndim < 3 || c->E1 == 8);
+  return e8 ? rd::col_max_threads<8>() : rd::col_max_threads<16>();
+}
 
 int pow2_above(int x) {
   int p = 1;
@@ -768,11 +789,11 @@ int fixed_shapes(rdcu_ctx *c, const char *spec) {
   const bool ok =
       rpb >= 1 && rpb * T <= 256 &&
       (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx) <= 160 * 1024 &&
-      (c->ndim < 2 || (tc >= 1 && tc * (Lmax / Emin) <= 512 &&
+      (c->ndim < 2 || (tc >= 1 && tc * (Lmax / Emin) <= col_threads(c) &&
                        (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) <=
                            160 * 1024 &&
                        fpw >= 1 && fpw <= std::max(1, c->max_fpf))) &&
-      G >= c->n && R >= 1 &&
+      G >= c->n && R >= 1 && (R == 1 || c->Ex <= 8) &&
       R * G * T <= 256 &&
       c->rows % R == 0 && rowi_lds(c, G, R) <= 160 * 1024;
   if (!ok) return fail(RDCU_EINVAL, std::string("RDCU_SHAPES out of range: ") + spec);
@@ -836,7 +857,7 @@ int autotune(rdcu_ctx *c) {
         Tmax = std::max(Tmax, c->Ns[1] / c->E1);
       }
       for (int tc = std::max(1, 64 / Tmax); tc <= 64; tc *= 2) {
-        if (tc * Tmax > 512 ||
+        if (tc * Tmax > col_threads(c) ||
             (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) > 160 * 1024)
           break;
         for (int fpw = c->max_fpf; fpw >= 1; fpw = fpw > 1 ? (fpw + 1) / 2 : 0) {
@@ -867,7 +888,7 @@ int autotune(rdcu_ctx *c) {
   best = 1e30;
   const int Gf = std::max(c->n, c->nf_used);
   for (int G = c->n; G * T <= 256; G = G < Gf ? G + 1 : (G == Gf ? pow2_above(Gf) : 2 * G)) {
-    for (int R = 1; R * G * T <= 256; R *= 2) {
+    for (int R = 1; R * G * T <= 256 && (R == 1 || c->Ex <= 8); R *= 2) {
       if (c->rows % R) break;
       if (rowi_lds(c, G, R) > 160 * 1024) break;
       double tm = time_of([&]() {

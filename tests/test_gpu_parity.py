@@ -294,3 +294,29 @@ def test_compiled_length_kernels_bitwise(hlib, olib, monkeypatch, Ns, Ls, shapes
             to, ho, ao = o.step(to, ho)
             assert (tg, hg, ag) == (to, ho, ao)
         assert np.array_equal(g.state(), o.state())
+
+
+def test_generated_kernels_compile_and_do_not_spill():
+    """The dictionary-specialised x-inverse kernels (and the 1-D attempt
+    kernel) are generated at run time; a generated kernel that keeps values
+    in scratch memory is rejected (the library then uses its interpreter).
+    For the bench shapes, short 3*2^k rows and 3-D, the generated kernels
+    must be in use: "rdcu jit: ok" for every handle."""
+    import sys
+    code = (
+        "import numpy as np\n"
+        "from rdcu_amd import Integrator, Problem, cgle_terms, glibc_rand_ic, hip_lib\n"
+        "lib = hip_lib()\n"
+        "for Ns in ([256], [48], [512, 512], [64, 192], [96, 96], [32, 32, 32]):\n"
+        "    nd = len(Ns); N = int(np.prod(Ns))\n"
+        "    print('case', Ns, flush=True)\n"
+        "    with Integrator(lib, Problem(2, Ns, [50.0] * nd, cgle_terms(nd)),\n"
+        "                    glibc_rand_ic(2, N, seed=1, A=1.0)) as it:\n"
+        "        pass\n")
+    from conftest import ROOT
+    env = dict(os.environ, RDCU_VERBOSE="1", PYTHONPATH=ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                       env=env, cwd=ROOT)
+    assert r.returncode == 0, r.stdout + r.stderr
+    status = [l for l in (r.stdout + r.stderr).split("\n") if l.startswith("rdcu jit:")]
+    assert len(status) == 6 and all(l == "rdcu jit: ok" for l in status), status
