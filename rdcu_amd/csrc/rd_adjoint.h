@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *
                                                   const int *__restrict__ tmap,
                                                   const double *__restrict__ u,
                                                   const double *__restrict__ W, double *out_id,
-                                                  double *Gk, double *P, int G) {
+                                                  double *Gk, double *P, int G, cx *Spc) {
   extern __shared__ cx smem[];
   const int n = g.n, Nx = g.Nx, L = g.L, T = L / E;
   const long N = g.N, rows = g.rows;
@@ -145,6 +145,49 @@ __global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *
     out_id[(long)(w / Nx) * N + r * Nx + (w % Nx)] = gacc[w];
   for (int w = tid; w < nf * Nx; w += nt)
     Gk[(long)(w / Nx) * N + r * Nx + (w % Nx)] = gacc[n * Nx + w];
+  // The x forward pass of the weight rows for the companion problem (its
+  // field m is weight row m), with the operations of rowf_body: the real row
+  // as L complex values, forward transform, then the split into the
+  // half spectrum, written to the companion's S.  Only with 8 values per
+  // thread: the 24-value kernel has no registers left for it (Spc is null).
+  if constexpr (E != 8) return;
+  if (!Spc) return;
+  const long Nxp = g.Nxp;
+  for (int g0 = 0; g0 < nf; g0 += G) {
+    const int m = g0 + gl;
+    const bool act = gl < G && m < nf;
+    const double *wr = gacc + (long)(n + (act ? m : 0)) * Nx;
+    __syncthreads();  // weights complete; lines free
+    cx v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const double2 o = *reinterpret_cast<const double2 *>(wr + 2 * (t + T * q));
+      v[q] = act ? cx{o.x, o.y} : cx{0.0, 0.0};
+    }
+    reg_fft<E, -1>(v, lb, 1, tw, px, t, T);
+#pragma unroll
+    for (int q = 0; q < E; ++q) lb[t + T * q] = v[q];
+    __syncthreads();
+    if (act) {
+      cx *S = Spc + ((long)m * rows + r) * Nxp;
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const int k = t + T * q;
+        const cx Av = v[q];
+        const cx Bv = cconj(lb[k == 0 ? 0 : L - k]);
+        const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
+        const cx tt = cmul(D, twN[k]);
+        S[k] = cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)};
+      }
+      if (t == 0) {
+        const cx Av = lb[0];
+        const cx Bv = cconj(lb[0]);
+        const cx Ev = cadd(Av, Bv), D = csub(Av, Bv);
+        const cx tt = cmul(D, twN[L]);
+        S[L] = cx{0.5 * (Ev.re + tt.im), 0.5 * (Ev.im - tt.re)};
+      }
+    }
+  }
 }
 
 // The seven stage values of the step just taken, into dst[j*nN + i]: every

@@ -119,18 +119,28 @@ int vjp(rdcu_ctx *c, const double *u, const double *w, double *out_id, double *G
                      (size_t)(2 * (c->n + nf) + c->n) * c->Nx * sizeof(double) +
                      (size_t)((c->dictp.nwords + 1) & ~1) * sizeof(int) + nred * sizeof(double);
   if (lds > 160 * 1024) return fail(RDCU_EUNSUPPORTED, "row too long for the adjoint kernel");
+  // the companion's x forward pass runs inside the row kernel (its S)
+  rdcu_ctx *pc = nf > 0 ? c->adj_pc.get() : nullptr;
+  cx *spc = pc && pc->reg && c->Ex == 8 ? pc->bufs.S : nullptr;
   if (c->Ex == 8)
     hipLaunchKernelGGL(rd::k_rowi_vjp<8>, dim3((unsigned)c->rows), dim3(nt), lds, c->stream,
                        c->grid, c->px, c->twN, c->bufs, c->fl_used, c->dictp, tmap, u, w, out_id,
-                       Gk, P, G);
+                       Gk, P, G, spc);
   else
     hipLaunchKernelGGL(rd::k_rowi_vjp<24>, dim3((unsigned)c->rows), dim3(nt), lds, c->stream,
                        c->grid, c->px, c->twN, c->bufs, c->fl_used, c->dictp, tmap, u, w, out_id,
-                       Gk, P, G);
+                       Gk, P, G, spc);
   HIPCHECK(hipGetLastError());
-  if (nf > 0) {
-    int st = enqueue_rhs(c->adj_pc.get(), 0, rd::SRC_PTR, Gk, rd::DST_PTR, 0, Dk, 0, 0);
+  if (pc && !spc) {
+    int st = enqueue_rhs(pc, 0, rd::SRC_PTR, Gk, rd::DST_PTR, 0, Dk, 0, 0);
     if (st) return st;
+  } else if (pc) {  // the companion's strided and x-inverse passes
+    auto ls = rhs_launches_reg(pc, rd::SRC_PTR, Gk, rd::DST_PTR, 0, Dk, 0);
+    if (ls.empty() || std::string(ls[0].name) != "row_fwd")
+      return fail(RDCU_EDEVICE, "companion pipeline does not start with its x forward pass");
+    for (size_t i = 1; i < ls.size(); ++i) ls[i].go();
+    HIPCHECK(hipGetLastError());
+    pc->nrhs++;
   }
   return RDCU_OK;
 }
