@@ -34,13 +34,24 @@ __device__ __forceinline__ double wave_sum(double v) {
 // whole waves, each term's partial sums are reduced within the wave and the
 // waves' sums added at the end (no barrier per term); otherwise by a tree in
 // LDS per term.
-template <int E>
-__global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *__restrict__ twN,
-                                                  Bufs b, FacList fl, DictP dict,
-                                                  const int *__restrict__ tmap,
-                                                  const double *__restrict__ u,
-                                                  const double *__restrict__ W, double *out_id,
-                                                  double *Gk, double *P, int G, cx *Spc) {
+// The per-term work (VD): ProgVjp interprets the dictionary program; the
+// generated JitVjp (hiprtc, one per dictionary structure) is the same
+// operations in the same order as straight-line code, with the slot values
+// and weights in registers; it needs whole waves.
+struct ProgVjp {
+  static constexpr bool kGen = false;
+  __device__ __forceinline__ static void run(const int *, const double *, const double *,
+                                             double *, double *, int, int, int, int) {}
+};
+
+template <int E, class VD>
+__device__ __forceinline__ void rowi_vjp_body(const Grid &g, const FftPlan &px,
+                                              const cx *__restrict__ twN, const Bufs &b,
+                                              const FacList &fl, const DictP &dict,
+                                              const int *__restrict__ tmap,
+                                              const double *__restrict__ u,
+                                              const double *__restrict__ W, double *out_id,
+                                              double *Gk, double *P, int G, cx *Spc) {
   extern __shared__ cx smem[];
   const int n = g.n, Nx = g.Nx, L = g.L, T = L / E;
   const long N = g.N, rows = g.rows;
@@ -87,7 +98,8 @@ __global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *
   }
   __syncthreads();
   // terms in program order (fields ascending, input order within a field)
-  for (int f = 0; f < n; ++f) {
+  if constexpr (VD::kGen) VD::run(prog, vals, wrow, gacc, red, Nx, tid, nt, nwv);
+  for (int f = 0; f < n && !VD::kGen; ++f) {
     int p = prog[f];
     const int pe = prog[f + 1];
     int pt = 0;  // program-order term index of field f's first term
@@ -188,6 +200,16 @@ __global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *
       }
     }
   }
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *__restrict__ twN,
+                                                  Bufs b, FacList fl, DictP dict,
+                                                  const int *__restrict__ tmap,
+                                                  const double *__restrict__ u,
+                                                  const double *__restrict__ W, double *out_id,
+                                                  double *Gk, double *P, int G, cx *Spc) {
+  rowi_vjp_body<E, ProgVjp>(g, px, twN, b, fl, dict, tmap, u, W, out_id, Gk, P, G, Spc);
 }
 
 // The seven stage values of the step just taken, into dst[j*nN + i]: every

@@ -62,10 +62,140 @@ int place_outputs(double t0, long long ns, const double *hs, int nout, const dou
   return RDCU_OK;
 }
 
+// Straight-line per-term work of the vector-Jacobian row kernel for one
+// dictionary structure: the operations of the interpreter in rowi_vjp_body
+// (per point: fields ascending, terms in input order, factors in order;
+// powers as repeated products; term partials reduced per wave), with the
+// slot values, weights and per-term partials in registers.
+std::string jit_vjp_source(int n, int nslots, const std::vector<int> &tfield,
+                           const std::vector<int> &tstart, const std::vector<int> &fslot,
+                           const std::vector<int> &fpow) {
+  auto powexpr = [](int sl, int p) {
+    const std::string v = "v" + std::to_string(sl);
+    if (p == 0) return std::string("1.0");
+    const int a = p < 0 ? -p : p;
+    std::string e = v;
+    for (int j = 1; j < a; ++j) e = "(" + e + " * " + v + ")";
+    return p < 0 ? "(1.0 / " + e + ")" : e;
+  };
+  const int nt = (int)tfield.size();
+  std::vector<int> order, coff;  // program order of the terms; coefficient word offsets
+  {
+    int pos = n + 1;
+    for (int f = 0; f < n; ++f)
+      for (int i = 0; i < nt; ++i)
+        if (tfield[i] == f) {
+          order.push_back(i);
+          coff.push_back(pos + 1);
+          pos += 3 + 2 * (tstart[i + 1] - tstart[i]);
+        }
+  }
+  std::vector<char> used(nslots, 0), grad(nslots, 0);
+  for (int i : order)
+    for (int j = tstart[i]; j < tstart[i + 1]; ++j) {
+      used[fslot[j]] = 1;
+      if (fpow[j] != 0) grad[fslot[j]] = 1;
+    }
+  char buf[256];
+  std::string s = "namespace rd {\nstruct JitVjp {\n  static constexpr bool kGen = true;\n"
+                  "  __device__ __forceinline__ static void run(const int *__restrict__ P, "
+                  "const double *vals, const double *wrow, double *gacc, double *red, int Nx, "
+                  "int tid, int nt, int nwv) {\n";
+  for (size_t q = 0; q < order.size(); ++q) {
+    snprintf(buf, sizeof buf, "    const double c%zu = __hiloint2double(P[%d], P[%d]);\n    double a%zu = 0.0;\n",
+             q, coff[q] + 1, coff[q], q);
+    s += buf;
+  }
+  s += "    for (int x = tid; x < Nx; x += nt) {\n";
+  for (int sl = 0; sl < nslots; ++sl) {
+    if (used[sl]) {
+      snprintf(buf, sizeof buf, "      const double v%d = vals[%d * Nx + x];\n", sl, sl);
+      s += buf;
+    }
+    if (grad[sl]) {
+      snprintf(buf, sizeof buf, "      double g%d = 0.0;\n", sl);
+      s += buf;
+    }
+  }
+  for (int f = 0, q = 0; f < n; ++f) {
+    snprintf(buf, sizeof buf, "      {\n        const double w = wrow[%d * Nx + x];\n", f);
+    s += buf;
+    for (; q < (int)order.size() && tfield[order[q]] == f; ++q) {
+      const int i = order[q];
+      s += "        { double t = 1.0;";
+      for (int j = tstart[i]; j < tstart[i + 1]; ++j) s += " t = t * " + powexpr(fslot[j], fpow[j]) + ";";
+      snprintf(buf, sizeof buf, " a%d = a%d + w * t;\n", q, q);
+      s += buf;
+      for (int j = tstart[i]; j < tstart[i + 1]; ++j) {
+        if (fpow[j] == 0) continue;
+        snprintf(buf, sizeof buf, "          { double d = %d.0 * %s;", fpow[j],
+                 powexpr(fslot[j], fpow[j] - 1).c_str());
+        s += buf;
+        for (int k = tstart[i]; k < tstart[i + 1]; ++k)
+          if (k != j) s += " d = d * " + powexpr(fslot[k], fpow[k]) + ";";
+        snprintf(buf, sizeof buf, " g%d = g%d + (w * c%d) * d; }\n", fslot[j], fslot[j], q);
+        s += buf;
+      }
+      s += "        }\n";
+    }
+    s += "      }\n";
+  }
+  for (int sl = 0; sl < nslots; ++sl)
+    if (grad[sl]) {
+      snprintf(buf, sizeof buf, "      gacc[%d * Nx + x] = g%d;\n", sl, sl);
+      s += buf;
+    }
+  s += "    }\n";
+  for (size_t q = 0; q < order.size(); ++q) {
+    snprintf(buf, sizeof buf,
+             "    { const double r_ = wave_sum(a%zu); if ((tid & 63) == 0) red[%zu * nwv + (tid >> 6)] = r_; }\n",
+             q, q);
+    s += buf;
+  }
+  s += "  }\n};\n}  // namespace rd\n";
+  return s;
+}
+
+// Compiles the vector-Jacobian row kernel for c's dictionary structure
+// (once per handle, on the first gradient call).  Without it (hiprtc
+// disabled or failing, or a kernel that would spill) the interpreter runs.
+void build_jit_vjp(rdcu_ctx *c) {
+  if (c->jit_vjp || !jit_enabled()) return;
+  std::string src = "#include \"rd_adjoint.h\"\n" +
+                    jit_vjp_source(c->n, c->n + c->nf_used, c->h_tfield, c->h_tstart, c->h_fslot,
+                                   c->h_fpow);
+  src += "extern \"C\" __global__ __launch_bounds__(256) void rdcu_rowi_vjp_jit(rd::Grid g, "
+         "rd::FftPlan px, const rd::cx *__restrict__ twN, rd::Bufs b, rd::FacList fl, rd::DictP dict, "
+         "const int *__restrict__ tmap, const double *__restrict__ u, const double *__restrict__ W, "
+         "double *out_id, double *Gk, double *P, int G, rd::cx *Spc) {\n"
+         "  rd::rowi_vjp_body<" + std::to_string(c->Ex) + ", rd::JitVjp>(g, px, twN, b, fl, dict, "
+         "tmap, u, W, out_id, Gk, P, G, Spc);\n}\n";
+  std::vector<char> code;
+  std::string log;
+  const char *vb = getenv("RDCU_VERBOSE");
+  if (!jit_compile(src, jit_arch(c->dev), code, log)) {
+    if (vb && *vb) fprintf(stderr, "rdcu vjp jit: %s\n", log.c_str());
+    return;
+  }
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  if (hipModuleLoadData(&mod, code.data()) != hipSuccess ||
+      hipModuleGetFunction(&fn, mod, "rdcu_rowi_vjp_jit") != hipSuccess || jit_spills(fn)) {
+    (void)hipGetLastError();
+    if (mod) (void)hipModuleUnload(mod);
+    if (vb && *vb) fprintf(stderr, "rdcu vjp jit: not used (load failed or spills)\n");
+    return;
+  }
+  c->jit_vjp_mod = mod;
+  c->jit_vjp = fn;
+  if (vb && *vb) fprintf(stderr, "rdcu vjp jit: ok\n");
+}
+
 // Companion problem: one field per referenced factor m of c, whose RHS is
 // sign_m * D_m applied to it (sign -1 for first derivatives: D^T = -D).
 int adj_companion(rdcu_ctx *c) {
   if (c->adj_pc) return RDCU_OK;
+  build_jit_vjp(c);
   const int nf = c->nf_used;
   if (nf > rd::kNMax)
     return fail(RDCU_EUNSUPPORTED, "gradients need at most 8 referenced derivative factors");
@@ -122,7 +252,22 @@ int vjp(rdcu_ctx *c, const double *u, const double *w, double *out_id, double *G
   // the companion's x forward pass runs inside the row kernel (its S)
   rdcu_ctx *pc = nf > 0 ? c->adj_pc.get() : nullptr;
   cx *spc = pc && pc->reg && c->Ex == 8 ? pc->bufs.S : nullptr;
-  if (c->Ex == 8)
+  if (c->jit_vjp && nt % 64 == 0) {
+    rd::Grid a_g = c->grid;
+    rd::FftPlan a_px = c->px;
+    const rd::cx *a_tw = c->twN;
+    rd::Bufs a_b = c->bufs;
+    rd::FacList a_fl = c->fl_used;
+    rd::DictP a_d = c->dictp;
+    const int *a_tm = tmap;
+    const double *a_u = u, *a_w = w;
+    double *a_oid = out_id, *a_gk = Gk, *a_p = P;
+    int a_G = G;
+    cx *a_s = spc;
+    void *args[] = {&a_g, &a_px, &a_tw, &a_b, &a_fl, &a_d, &a_tm, &a_u, &a_w, &a_oid, &a_gk, &a_p, &a_G, &a_s};
+    HIPCHECK(hipModuleLaunchKernel(c->jit_vjp, (unsigned)c->rows, 1, 1, (unsigned)nt, 1, 1,
+                                   (unsigned)lds, c->stream, args, nullptr));
+  } else if (c->Ex == 8)
     hipLaunchKernelGGL(rd::k_rowi_vjp<8>, dim3((unsigned)c->rows), dim3(nt), lds, c->stream,
                        c->grid, c->px, c->twN, c->bufs, c->fl_used, c->dictp, tmap, u, w, out_id,
                        Gk, P, G, spc);

@@ -134,6 +134,9 @@ struct rdcu_ctx {
   // 1-D: a whole attempt in one workgroup (null if not built or not eligible)
   hipFunction_t jit_att = nullptr;
   size_t lds_att = 0;
+  // gradients: the vector-Jacobian row kernel with the dictionary compiled in
+  hipModule_t jit_vjp_mod = nullptr;
+  hipFunction_t jit_vjp = nullptr;
   std::string jit_log;
   std::vector<int> h_tfield, h_tstart, h_fslot, h_fpow;
   std::vector<double> h_tcoef;
@@ -174,6 +177,7 @@ struct rdcu_ctx {
   ~rdcu_ctx() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (jit_mod) (void)hipModuleUnload(jit_mod);
+    if (jit_vjp_mod) (void)hipModuleUnload(jit_vjp_mod);
     if (graph) (void)hipGraphDestroy(graph);
     for (void *p : allocs) (void)hipFree(p);
     adj_pc.reset();

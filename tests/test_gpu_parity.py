@@ -298,10 +298,10 @@ def test_compiled_length_kernels_bitwise(hlib, olib, monkeypatch, Ns, Ls, shapes
 
 def test_generated_kernels_compile_and_do_not_spill():
     """The dictionary-specialised x-inverse kernels (and the 1-D attempt
-    kernel) are generated at run time; a generated kernel that keeps values
-    in scratch memory is rejected (the library then uses its interpreter).
-    For the bench shapes, short 3*2^k rows and 3-D, the generated kernels
-    must be in use: "rdcu jit: ok" for every handle."""
+    kernel, and the gradient's row kernel) are generated at run time; a
+    generated kernel that keeps values in scratch memory is rejected (the
+    library then uses its interpreter).  For the bench shapes, short 3*2^k
+    rows and 3-D, the generated kernels must be in use."""
     import sys
     code = (
         "import numpy as np\n"
@@ -312,11 +312,16 @@ def test_generated_kernels_compile_and_do_not_spill():
         "    print('case', Ns, flush=True)\n"
         "    with Integrator(lib, Problem(2, Ns, [50.0] * nd, cgle_terms(nd)),\n"
         "                    glibc_rand_ic(2, N, seed=1, A=1.0)) as it:\n"
-        "        pass\n")
+        "        if nd == 2 and Ns[0] == 64:  # the gradient kernel is built on first use\n"
+        "            y0 = glibc_rand_ic(2, N, seed=1, A=1.0)\n"
+        "            it.loss_grad(y0, 0.0, np.full(3, 1e-3), [2e-3], np.zeros(2 * N))\n")
     from conftest import ROOT
     env = dict(os.environ, RDCU_VERBOSE="1", PYTHONPATH=ROOT)
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
                        env=env, cwd=ROOT)
     assert r.returncode == 0, r.stdout + r.stderr
-    status = [l for l in (r.stdout + r.stderr).split("\n") if l.startswith("rdcu jit:")]
-    assert len(status) == 6 and all(l == "rdcu jit: ok" for l in status), status
+    out = (r.stdout + r.stderr).split("\n")
+    status = [l for l in out if l.startswith("rdcu jit:")]
+    # 6 handles, plus the gradient's companion problem
+    assert len(status) == 7 and all(l == "rdcu jit: ok" for l in status), status
+    assert [l for l in out if l.startswith("rdcu vjp jit:")] == ["rdcu vjp jit: ok"]
