@@ -233,11 +233,16 @@ int adj_companion(rdcu_ctx *c) {
 
 // J(u)^T w -> out_id (identity slots) and Dk (derivative blocks, signed);
 // coefficient partials added into P.
+// pre: u's x-forward half spectra and factor spectra already computed
+// (S, F of *pre); null: computed here into c's buffers.
 int vjp(rdcu_ctx *c, const double *u, const double *w, double *out_id, double *Gk, double *Dk,
-        double *P, const int *tmap) {
+        double *P, const int *tmap, const rd::Bufs *pre = nullptr) {
   rd::StageCoef none{};
-  rowf(c, 0, rd::SRC_PTR, u, none, c->rf_rpb);
-  for (auto &l : col_launches_reg(c, 0, c->cr_tc)) l.go();
+  if (!pre) {
+    rowf(c, 0, rd::SRC_PTR, u, none, c->rf_rpb);
+    for (auto &l : col_launches_reg(c, 0, c->cr_tc)) l.go();
+  }
+  const rd::Bufs bv = pre ? *pre : c->bufs;
   const int nf = c->nf_used, T = c->L / c->Ex;
   // all 256 threads when the line groups divide them (idle lines, but the
   // per-term loop over the row's points gets the most threads)
@@ -256,7 +261,7 @@ int vjp(rdcu_ctx *c, const double *u, const double *w, double *out_id, double *G
     rd::Grid a_g = c->grid;
     rd::FftPlan a_px = c->px;
     const rd::cx *a_tw = c->twN;
-    rd::Bufs a_b = c->bufs;
+    rd::Bufs a_b = bv;
     rd::FacList a_fl = c->fl_used;
     rd::DictP a_d = c->dictp;
     const int *a_tm = tmap;
@@ -269,11 +274,11 @@ int vjp(rdcu_ctx *c, const double *u, const double *w, double *out_id, double *G
                                    (unsigned)lds, c->stream, args, nullptr));
   } else if (c->Ex == 8)
     hipLaunchKernelGGL(rd::k_rowi_vjp<8>, dim3((unsigned)c->rows), dim3(nt), lds, c->stream,
-                       c->grid, c->px, c->twN, c->bufs, c->fl_used, c->dictp, tmap, u, w, out_id,
+                       c->grid, c->px, c->twN, bv, c->fl_used, c->dictp, tmap, u, w, out_id,
                        Gk, P, G, spc);
   else
     hipLaunchKernelGGL(rd::k_rowi_vjp<24>, dim3((unsigned)c->rows), dim3(nt), lds, c->stream,
-                       c->grid, c->px, c->twN, c->bufs, c->fl_used, c->dictp, tmap, u, w, out_id,
+                       c->grid, c->px, c->twN, bv, c->fl_used, c->dictp, tmap, u, w, out_id,
                        Gk, P, G, spc);
   HIPCHECK(hipGetLastError());
   if (pc && !spc) {
@@ -435,10 +440,14 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
   const int nf = c->nf_used, nt = (int)c->h_tcoef.size();
   double *K[7], *Ys[7], *kbar[7], *ybar, *lam, *out_id, *Gk, *Dk, *P;
   int *tmap, *fmap;
-  for (int j = 0; j < 7; ++j)
-    if ((!kst && (st = ds.get(&K[j], nN))) || (st = ds.get(&Ys[j], nN)) ||
-        (st = ds.get(&kbar[j], nN)))
-      return st;
+  // the seven stage inputs side by side (stage s at Ys[s-1]; Ys[0] is the
+  // checkpoint's copy): their spectra are computed together, as 7n fields
+  double *Yall;
+  if ((st = ds.get(&Yall, (size_t)7 * nN))) return st;
+  for (int j = 0; j < 7; ++j) {
+    Ys[j] = Yall + (size_t)j * nN;
+    if ((!kst && (st = ds.get(&K[j], nN))) || (st = ds.get(&kbar[j], nN))) return st;
+  }
   if ((st = ds.get(&ybar, nN)) || (st = ds.get(&lam, nN)) || (st = ds.get(&out_id, nN)) ||
       (st = ds.get(&Gk, (size_t)nf * N)) || (st = ds.get(&Dk, (size_t)nf * N)) ||
       (st = ds.get(&P, (size_t)std::max(1, nt) * c->rows)) || (st = ds.get(&tmap, nt)) ||
@@ -477,9 +486,57 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
   double *d_oc, *ycur;
   long long *kdev;
   if ((st = ds.get(&d_obeg, ns)) || (st = ds.get(&d_oend, ns)) ||
-      (st = ds.get(&d_oc, (size_t)7 * nout)) || (st = ds.get(&ycur, nN)) ||
-      (st = ds.get(&kdev, 1)))
+      (st = ds.get(&d_oc, (size_t)7 * nout)) || (st = ds.get(&kdev, 1)))
     return st;
+  ycur = Ys[0];
+  // Spectra of the 7 stage inputs in one x-forward and one strided pipeline:
+  // the problem seen as 7n fields (field s*n+f = field f of stage s+1) with
+  // the factor list repeated per stage; stage s's spectra at offset s-1.
+  cx *S7, *F7;
+  int *fl7b, *fl7f, *fl7s;
+  const int nf7 = 7 * nf;
+  std::vector<int> h7b(std::max(1, nf7)), h7f(std::max(1, nf7)), h7s(7 * c->n + 1);
+  {
+    std::vector<int> ust(c->n + 1, 0);
+    for (int m = 0; m < nf; ++m) ust[c->h_ufld[m] + 1]++;
+    for (int f = 0; f < c->n; ++f) ust[f + 1] += ust[f];
+    for (int q = 0; q < 7; ++q) {
+      for (int m = 0; m < nf; ++m) {
+        h7b[q * nf + m] = c->h_ublk[m];
+        h7f[q * nf + m] = q * c->n + c->h_ufld[m];
+      }
+      for (int f = 0; f < c->n; ++f) h7s[q * c->n + f] = q * nf + ust[f];
+    }
+    h7s[7 * c->n] = nf7;
+  }
+  if ((st = ds.get(&S7, (size_t)7 * c->n * c->Nh)) ||
+      (st = ds.get(&F7, (size_t)std::max(1, nf7) * c->Nh)) || (st = ds.get(&fl7b, h7b.size())) ||
+      (st = ds.get(&fl7f, h7f.size())) || (st = ds.get(&fl7s, h7s.size())))
+    return st;
+  HIPCHECK(hipMemcpyAsync(fl7b, h7b.data(), h7b.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(fl7f, h7f.data(), h7f.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(fl7s, h7s.data(), h7s.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  const bool batched_spectra = c->ndim >= 2;  // in 1-D the row kernel reads S only
+  auto enqueue_spectra7 = [&]() {
+    const rd::Grid g0 = c->grid;
+    const rd::Bufs b0 = c->bufs;
+    const rd::FacList f0 = c->fl_used;
+    const int n0 = c->n, nf0 = c->nf_used;
+    c->grid.n = 7 * n0;
+    c->n = 7 * n0;
+    c->nf_used = nf7;
+    c->fl_used = rd::FacList{nf7, fl7b, fl7f, fl7s};
+    c->bufs.S = S7;
+    c->bufs.F = F7;
+    rd::StageCoef none{};
+    rowf(c, 0, rd::SRC_PTR, Yall, none, c->rf_rpb);
+    for (auto &l : col_launches_reg(c, 0, c->cr_tc)) l.go();
+    c->grid = g0;
+    c->bufs = b0;
+    c->fl_used = f0;
+    c->n = n0;
+    c->nf_used = nf0;
+  };
   HIPCHECK(hipMemcpyAsync(d_obeg, obeg.data(), ns * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemcpyAsync(d_oend, oend.data(), ns * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemcpyAsync(d_oc, oc.data(), oc.size() * sizeof(double), hipMemcpyHostToDevice,
@@ -506,6 +563,7 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
       if (!kst && (s2 = enqueue_rhs(c, 0, rd::SRC_PTR, Ys[s - 1], rd::DST_PTR, 0, K[s - 1], 0, 0)))
         return s2;
     }
+    if (batched_spectra) enqueue_spectra7();
     rd::AdjK kz{};
     for (int j = 0; j < 7; ++j) kz.k[j] = kbar[j];
     hipLaunchKernelGGL(rd::k_adj_zero, dim3(nb), dim3(256), 0, c->stream, kz, ybar_out, nN);
@@ -514,7 +572,11 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
                        (const double *)d_oc, (const long long *)kdev, nN);
     for (int s = 7; s >= 1; --s) {
       const double *u = s == 1 ? ycur : Ys[s - 1];
-      if ((s2 = vjp(c, u, kbar[s - 1], out_id, Gk, Dk, P, tmap))) return s2;
+      rd::Bufs pre = c->bufs;
+      pre.S = S7 + (size_t)(s - 1) * c->n * c->Nh;
+      pre.F = F7 + (size_t)(s - 1) * nf * c->Nh;
+      if ((s2 = vjp(c, u, kbar[s - 1], out_id, Gk, Dk, P, tmap, batched_spectra ? &pre : nullptr)))
+        return s2;
       rd::AdjK kb{};
       for (int j = 0; j < 7; ++j) kb.k[j] = kbar[j];
       if (s >= 2)
