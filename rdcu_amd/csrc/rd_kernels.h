@@ -539,16 +539,52 @@ __global__ void k_reduce(const double *__restrict__ in, long cnt, double *out, D
   if (threadIdx.x == 0) out[blockIdx.x] = v[0];
 }
 
+// The pairwise tree of lds_pairsum over part[0..cnt), cnt <= 4096, in a
+// 256-thread block without a barrier per level: thread i sums its 16 values
+// part[16i..16i+15] pairwise in registers, the lanes of a wave combine by
+// butterfly shuffles (at distance d, lane x adds lane x+d's sum to its own:
+// the tree's left + right; the partner computes right + left, the same bits
+// since addition commutes), and the block's four waves through LDS.  Values
+// past cnt count as +0, which leaves these sums unchanged: the partials are
+// sums of squares (>= +0, or NaN).  Every thread calls it; the result is
+// valid in thread 0.  lds: 4 doubles.
+__device__ __forceinline__ double tree_sum256(const double *__restrict__ part, int cnt,
+                                              double *lds) {
+  const int tid = threadIdx.x;
+  double v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int i = 16 * tid + q;
+    v[q] = i < cnt ? part[i] : 0.0;
+  }
+#pragma unroll
+  for (int s = 1; s < 16; s <<= 1)
+#pragma unroll
+    for (int q = 0; q < 16; q += 2 * s) v[q] = v[q] + v[q + s];
+  double a = v[0];
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) a = a + __shfl_xor(a, d, 64);
+  if ((tid & 63) == 0) lds[tid >> 6] = a;
+  __syncthreads();
+  return tid == 0 ? (lds[0] + lds[1]) + (lds[2] + lds[3]) : 0.0;
+}
+
 // Error norm and step control (dp45.cu:117-146) on the device.
 // v: LDS for cnt doubles.  Every thread of the block calls it.
 __device__ __forceinline__ void control_step(DevState *st, const double *__restrict__ part,
                                              long cnt, long nN, double *v) {
+  // with 256 threads the partials are summed by tree_sum256, their loads
+  // issued together with the state's
+  const bool fast = blockDim.x == 256 && cnt <= 4096;
+  double sum = 0.0;
+  if (fast) sum = tree_sum256(part, (int)cnt, v);
   if (!attempt_active(st)) return;
   const int fixed = st->fixed;
-  if (!fixed) {
+  if (!fixed && !fast) {
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) v[i] = part[i];
     __syncthreads();
     lds_pairsum(v, (int)cnt, threadIdx.x, blockDim.x);
+    sum = v[0];
   }
   if (threadIdx.x != 0) return;
   double t = st->t, h = st->h;
@@ -557,7 +593,7 @@ __device__ __forceinline__ void control_step(DevState *st, const double *__restr
   if (fixed) {
     acc = true;
   } else {
-    norm = sqrt(v[0]) / sqrt((double)nN);
+    norm = sqrt(sum) / sqrt((double)nN);
     factor = 0.9 * pow_m02(norm);
     if (__builtin_isnan(factor)) factor = 0.2;
     acc = norm < 1;
