@@ -287,8 +287,8 @@ def main():
                     help="instances of the batched adjoint on the GPU together")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each instance lives "
-              "on one GPU; see DESIGN.md for how a run would shard). Use --gpus 1.",
+        print("bench.py measures one GPU: there is no multi-GPU path here (each integrator "
+              "instance lives on one GPU). Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
