@@ -260,7 +260,9 @@ __global__ __launch_bounds__(256) void k_rowf_reg(Grid g, FftPlan px, const cx *
 // Strided-line pass (modes as k_col).  Block: tc columns x T threads; LDS:
 // twiddles L + tc*padded_len(L) complex (element e of column c at
 // buf[pad8(e)*tc + c]) + tc*L complex holding the forward spectrum (mode 1).
-template <int E, bool NT = false, int LS = 0>
+// TG: twiddles and axis-0 wavenumbers read from global memory instead of LDS
+// copies (long lines, where the copies would leave room for one column).
+template <int E, bool NT = false, int LS = 0, bool TG = false>
 __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan &p,
                                          const ColGeom &cg, const Bufs &b, const FacList &fl,
                                          int mode, int fpw) {
@@ -285,9 +287,11 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   const int tid = threadIdx.x;
   const int c = tid % tc, t = tid / tc;
   const bool act = c < nc;
-  cx *tw = smem, *buf = smem + L;
+  cx *twl = smem, *buf = TG ? smem : smem + L;
+  const cx *tw = TG ? p.tw : twl;
   // mode 1: wavenumbers of axis 0 after the forward spectrum in LDS
-  double *k0s = reinterpret_cast<double *>(buf + tc * padded_len(L) + tc * L);
+  double *k0l = reinterpret_cast<double *>(buf + tc * padded_len(L) + tc * L);
+  const double *k0s = TG ? g.kt[0] : k0l;
   const long lbase = (long)g2 * cg.gs2 + c0 + c;
   cx *src = (mode == 2 ? b.F : b.S) + (long)g1 * cg.gs1 + lbase;
   cx v[E];
@@ -312,12 +316,14 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     }
     blk0 = fl.fblk[m0];
   }
-  Prefetch<4, cx> pf_tw;
-  pf_tw.load(p.tw, L, tid, blockDim.x);
-  Prefetch<4, double> pf_k;
-  if (mode == 1) pf_k.load(g.kt[0], L, tid, blockDim.x);
-  pf_tw.store(tw, p.tw, L, tid, blockDim.x);
-  if (mode == 1) pf_k.store(k0s, g.kt[0], L, tid, blockDim.x);
+  if constexpr (!TG) {
+    Prefetch<4, cx> pf_tw;
+    pf_tw.load(p.tw, L, tid, blockDim.x);
+    Prefetch<4, double> pf_k;
+    if (mode == 1) pf_k.load(g.kt[0], L, tid, blockDim.x);
+    pf_tw.store(twl, p.tw, L, tid, blockDim.x);
+    if (mode == 1) pf_k.store(k0l, g.kt[0], L, tid, blockDim.x);
+  }
   cx *lb = buf + c;
   if (mode != 1) {
     if (mode == 0) fft_any<E, -1, LS>(v, lb, tc, tw, p, t, T);
@@ -358,14 +364,14 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
 template <int E>
 constexpr int col_max_threads() { return E == 8 ? 512 : 256; }
 
-template <int E, bool NT, int LS = 0>
+template <int E, bool NT, int LS = 0, bool TG = false>
 __global__ __launch_bounds__(col_max_threads<E>()) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
                                                  int check_active, Bufs b, FacList fl, int mode,
                                                  int fpw) {
   if (check_active && !attempt_active(st)) return;
   const int lblk = xcd_block(blockIdx.x, gridDim.x);
   if (lblk >= cg.nblk) return;
-  col_body<E, NT, LS>(lblk, g, p, cg, b, fl, mode, fpw);
+  col_body<E, NT, LS, TG>(lblk, g, p, cg, b, fl, mode, fpw);
 }
 
 // Inverse x pass for one row position r, all fields: factor lines in groups

@@ -376,12 +376,33 @@ void go_col(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, const rd::FftPlan &p, const
                      check, c->bufs, fl, mode, fpw);
 }
 
+// Strided lines of at least this length with 8 values per thread read their
+// twiddles and wavenumbers from global memory (k_col_reg<8, NT, 0, true>):
+// the LDS copies would leave room for a single column per block.
+constexpr int kColTgMin = 1024;
+bool col_tg(int L, int E) { return E == 8 && L >= kColTgMin; }
+
+// LDS of a strided-pass block of tc columns of length L (mode 1: with the
+// kept spectrum and the wavenumbers).
+size_t col_lds(int L, int tc, int E, bool mode1) {
+  const bool tg = col_tg(L, E);
+  return (size_t)((tg ? 0 : L) + tc * rd::padded_len(L) + (mode1 ? tc * L : 0)) * sizeof(cx) +
+         (mode1 && !tg ? (size_t)L * sizeof(double) : 0);
+}
+
 template <int E, bool NT>
 void go_col_l(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, const rd::FftPlan &p, const rd::ColGeom &cg,
               int check, const rd::FacList &fl, int mode, int fpw) {
   if constexpr (E == 8 || E == 16) {
     if (cg.L == kColLs[0]) return go_col<E, NT, kColLs[0]>(c, g, b, lds, p, cg, check, fl, mode, fpw);
     if (cg.L == kColLs[1]) return go_col<E, NT, kColLs[1]>(c, g, b, lds, p, cg, check, fl, mode, fpw);
+  }
+  if constexpr (E == 8) {
+    if (col_tg(cg.L, E)) {
+      hipLaunchKernelGGL((rd::k_col_reg<8, NT, 0, true>), g, b, lds, c->stream, c->grid, p, cg,
+                         c->st, check, c->bufs, fl, mode, fpw);
+      return;
+    }
   }
   go_col<E, NT, 0>(c, g, b, lds, p, cg, check, fl, mode, fpw);
 }
@@ -395,9 +416,7 @@ void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups
   if (cg.nparts < 1) cg.nparts = 1;
   cg.nblk = (int)(ngroups * cg.ntc * cg.nparts);
   dim3 g((unsigned)((cg.nblk + 7) / 8 * 8));
-  size_t lds = (size_t)(cg.L + cg.tc * rd::padded_len(cg.L) + (mode == 1 ? cg.tc * cg.L : 0)) *
-                   sizeof(cx) +
-               (mode == 1 ? (size_t)cg.L * sizeof(double) : 0);
+  const size_t lds = col_lds(cg.L, cg.tc, E, mode == 1);
   if (c->grid.nt_store)
     go_col_l<E, true>(c, g, dim3(cg.tc * T), lds, p, cg, check, fl, mode, fpw);
   else
@@ -761,6 +780,14 @@ void build_jit(rdcu_ctx *c) {
 // Shapes given by the caller instead of the autotuner (profiling runs that
 // must see only the launches of one shape), checked against the same limits.
 
+// Whether every strided-pass launch with tc columns per block fits the LDS
+// of a CU (axis 0 keeps the forward spectrum; in 3-D axis 1 does not).
+bool col_fits(const rdcu_ctx *c, int tc) {
+  size_t lds = col_lds(c->Ns[0], tc, c->E0, true);
+  if (c->ndim == 3) lds = std::max(lds, col_lds(c->Ns[1], tc, c->E1, false));
+  return lds <= 160 * 1024;
+}
+
 // Largest strided-pass block for the values per thread in use (the
 // kernels' launch bounds).
 int col_threads(const rdcu_ctx *c) {
@@ -794,8 +821,7 @@ int fixed_shapes(rdcu_ctx *c, const char *spec) {
       rpb >= 1 && rpb * T <= 256 &&
       (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx) <= 160 * 1024 &&
       (c->ndim < 2 || (tc >= 1 && tc * (Lmax / Emin) <= col_threads(c) &&
-                       (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) <=
-                           160 * 1024 &&
+                       col_fits(c, tc) &&
                        fpw >= 1 && fpw <= std::max(1, c->max_fpf))) &&
       G >= c->n && R >= 1 && (R == 1 || c->Ex <= 8) &&
       R * G * T <= 256 &&
@@ -862,7 +888,7 @@ int autotune(rdcu_ctx *c) {
       }
       for (int tc = std::max(1, 64 / Tmax); tc <= 64; tc *= 2) {
         if (tc * Tmax > col_threads(c) ||
-            (size_t)(Lmax + tc * rd::padded_len(Lmax) + tc * Lmax) * sizeof(cx) > 160 * 1024)
+            !col_fits(c, tc))
           break;
         for (int fpw = c->max_fpf; fpw >= 1; fpw = fpw > 1 ? (fpw + 1) / 2 : 0) {
           c->cr_fpw = fpw;
@@ -937,7 +963,9 @@ int allow_big_lds() {
       (const void *)rd::k_col_reg<16, false, kColLs[0]>,
       (const void *)rd::k_col_reg<16, true, kColLs[0]>,
       (const void *)rd::k_col_reg<16, false, kColLs[1]>,
-      (const void *)rd::k_col_reg<16, true, kColLs[1]>};
+      (const void *)rd::k_col_reg<16, true, kColLs[1]>,
+      (const void *)rd::k_col_reg<8, false, 0, true>,
+      (const void *)rd::k_col_reg<8, true, 0, true>};
   for (const void *f : fs)
     HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return RDCU_OK;

@@ -47,6 +47,9 @@ RHS_CASES = [
     ("3d-10x12x8-lds", [10, 12, 8], [5.0, 6.0, 4.0], "cgle"),
     ("2d-32x64-empty", [32, 64], [20.0, 30.0], "empty"),      # no terms: f = 0
     ("2d-64x32-onefield", [64, 32], [30.0, 20.0], "onefield"),  # field 1 has no terms
+    ("2d-1024x32", [1024, 32], [300.0, 20.0], "cgle"),     # long strided lines: twiddles from
+    ("2d-1024x16", [1024, 16], [300.0, 10.0], "mixed"),    # global memory in the strided pass
+    ("3d-1024x8x16", [1024, 8, 16], [300.0, 8.0, 12.0], "cgle"),
 ]
 
 
