@@ -289,8 +289,10 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   const bool act = c < nc;
   cx *twl = smem, *buf = TG ? smem : smem + L;
   const cx *tw = TG ? p.tw : twl;
-  // mode 1: wavenumbers of axis 0 after the forward spectrum in LDS
-  double *k0l = reinterpret_cast<double *>(buf + tc * padded_len(L) + tc * L);
+  // mode 1: the forward spectrum is kept in LDS when the block has several
+  // factors; the wavenumbers of axis 0 follow it
+  const bool keep = m1 - m0 > 1;
+  double *k0l = reinterpret_cast<double *>(buf + tc * padded_len(L) + (keep ? tc * L : 0));
   const double *k0s = TG ? g.kt[0] : k0l;
   const long lbase = (long)g2 * cg.gs2 + c0 + c;
   cx *src = (mode == 2 ? b.F : b.S) + (long)g1 * cg.gs1 + lbase;
@@ -335,6 +337,22 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     return;
   }
   fft_any<E, -1, LS>(v, lb, tc, tw, p, t, T);
+  if (!keep) {  // one factor: the spectrum goes from registers to its multiply
+    cx w[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      nk[0] = t + T * q;
+      kv[0] = k0s[t + T * q];
+      w[q] = mult_k(g, blk0, nk, kv, v[q]);
+    }
+    fft_any<E, +1, LS>(w, lb, tc, tw, p, t, T);
+    if (act) {
+      cx *dst = b.F + (long)m0 * g.Nh + lbase;
+#pragma unroll
+      for (int q = 0; q < E; ++q) st_g<NT>(dst + (long)(t + T * q) * cg.es, w[q]);
+    }
+    return;
+  }
   // keep the forward spectrum in LDS (each thread reloads only its own
   // elements, so no barrier is needed) instead of registers across factors
   cx *xs = buf + tc * padded_len(L);

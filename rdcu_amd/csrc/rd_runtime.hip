@@ -383,10 +383,11 @@ constexpr int kColTgMin = 1024;
 bool col_tg(int L, int E) { return E == 8 && L >= kColTgMin; }
 
 // LDS of a strided-pass block of tc columns of length L (mode 1: with the
-// kept spectrum and the wavenumbers).
-size_t col_lds(int L, int tc, int E, bool mode1) {
-  const bool tg = col_tg(L, E);
-  return (size_t)((tg ? 0 : L) + tc * rd::padded_len(L) + (mode1 ? tc * L : 0)) * sizeof(cx) +
+// wavenumbers, and with the kept spectrum when a block takes several
+// factors, fpw > 1).
+size_t col_lds(int L, int tc, int E, bool mode1, int fpw) {
+  const bool tg = col_tg(L, E), keep = mode1 && fpw > 1;
+  return (size_t)((tg ? 0 : L) + tc * rd::padded_len(L) + (keep ? tc * L : 0)) * sizeof(cx) +
          (mode1 && !tg ? (size_t)L * sizeof(double) : 0);
 }
 
@@ -416,7 +417,7 @@ void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups
   if (cg.nparts < 1) cg.nparts = 1;
   cg.nblk = (int)(ngroups * cg.ntc * cg.nparts);
   dim3 g((unsigned)((cg.nblk + 7) / 8 * 8));
-  const size_t lds = col_lds(cg.L, cg.tc, E, mode == 1);
+  const size_t lds = col_lds(cg.L, cg.tc, E, mode == 1, fpw);
   if (c->grid.nt_store)
     go_col_l<E, true>(c, g, dim3(cg.tc * T), lds, p, cg, check, fl, mode, fpw);
   else
@@ -782,9 +783,9 @@ void build_jit(rdcu_ctx *c) {
 
 // Whether every strided-pass launch with tc columns per block fits the LDS
 // of a CU (axis 0 keeps the forward spectrum; in 3-D axis 1 does not).
-bool col_fits(const rdcu_ctx *c, int tc) {
-  size_t lds = col_lds(c->Ns[0], tc, c->E0, true);
-  if (c->ndim == 3) lds = std::max(lds, col_lds(c->Ns[1], tc, c->E1, false));
+bool col_fits(const rdcu_ctx *c, int tc, int fpw) {
+  size_t lds = col_lds(c->Ns[0], tc, c->E0, true, fpw);
+  if (c->ndim == 3) lds = std::max(lds, col_lds(c->Ns[1], tc, c->E1, false, 1));
   return lds <= 160 * 1024;
 }
 
@@ -821,7 +822,7 @@ int fixed_shapes(rdcu_ctx *c, const char *spec) {
       rpb >= 1 && rpb * T <= 256 &&
       (size_t)(c->L + rpb * rd::padded_len(c->L)) * sizeof(cx) <= 160 * 1024 &&
       (c->ndim < 2 || (tc >= 1 && tc * (Lmax / Emin) <= col_threads(c) &&
-                       col_fits(c, tc) &&
+                       col_fits(c, tc, fpw) &&
                        fpw >= 1 && fpw <= std::max(1, c->max_fpf))) &&
       G >= c->n && R >= 1 && (R == 1 || c->Ex <= 8) &&
       R * G * T <= 256 &&
@@ -888,9 +889,10 @@ int autotune(rdcu_ctx *c) {
       }
       for (int tc = std::max(1, 64 / Tmax); tc <= 64; tc *= 2) {
         if (tc * Tmax > col_threads(c) ||
-            !col_fits(c, tc))
+            !col_fits(c, tc, 1))
           break;
         for (int fpw = c->max_fpf; fpw >= 1; fpw = fpw > 1 ? (fpw + 1) / 2 : 0) {
+          if (!col_fits(c, tc, fpw)) continue;
           c->cr_fpw = fpw;
           auto ls = col_launches_reg(c, 0, tc);
           double tm = time_of([&]() { for (auto &l : ls) l.go(); });
