@@ -337,7 +337,7 @@ int rdcu_schedule(rdcu_ctx *c, double t0, double h0, double hmin, double hmax, d
   while (tc < t1 && hc > hmin) {
     const long long K = kChunkMax;
     if ((st = begin(c, INFINITY, 0, 0, 0, hmin, hmax, t1))) return st;
-    for (long long i = 0; i < K; ++i) HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+    if ((st = launch_attempts(c, K))) return st;
     if ((st = read_state(c))) return st;
     const int nl = c->sh->nlog;
     c->nrhs += 6L * nl;

@@ -109,6 +109,7 @@ const double *const STAGE_A[6] = {A2, A3, A4, A5, A6, B7};
 const double kPI = 3.14159265358979323846;  // the reference's PI (rdcu.cu:14)
 constexpr size_t kLdsMax = 64 * 1024;
 constexpr int kChunkMax = 64;
+constexpr int kGraphAttempts = 8;  // attempts per multi-attempt graph; divides kChunkMax
 
 }  // namespace
 
@@ -149,6 +150,10 @@ struct rdcu_ctx {
   std::vector<void *> allocs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // kGraphAttempts attempts in one graph: fewer graph launches, and no gap
+  // between the graphs of consecutive attempts inside it
+  hipGraph_t graph_m = nullptr;
+  hipGraphExec_t gexec_m = nullptr;
   // launch shapes
   int r2c_rpb = 1, r2c_thr = 64;
   int col_tc = 1, col_thr = 64;
@@ -176,6 +181,8 @@ struct rdcu_ctx {
 
   ~rdcu_ctx() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (gexec_m) (void)hipGraphExecDestroy(gexec_m);
+    if (graph_m) (void)hipGraphDestroy(graph_m);
     if (jit_mod) (void)hipModuleUnload(jit_mod);
     if (jit_vjp_mod) (void)hipModuleUnload(jit_vjp_mod);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1056,6 +1063,18 @@ int enqueue_attempt(rdcu_ctx *c) {
   return RDCU_OK;
 }
 
+// Enqueues K attempts: whole groups through the multi-attempt graph, the rest
+// one graph each. Every attempt checks on the device whether it still runs, so
+// the grouping does not change which attempts take effect.
+int launch_attempts(rdcu_ctx *c, long long K) {
+  long long i = 0;
+  if (c->gexec_m)
+    for (; i + kGraphAttempts <= K; i += kGraphAttempts)
+      HIPCHECK(hipGraphLaunch(c->gexec_m, c->stream));
+  for (; i < K; ++i) HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+  return RDCU_OK;
+}
+
 int read_state(rdcu_ctx *c) {
   HIPCHECK(hipMemcpyAsync(c->sh, c->st, sizeof(rd::DevState), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1437,6 +1456,17 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
   if (ce != hipSuccess) return fail(RDCU_EDEVICE, std::string("capture: ") + hipGetErrorString(ce));
   c->graph = graph;
   HIPCHECK(hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0));
+  if (!c->member) {  // a batch member's attempts run through its batch's graph
+    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    st = RDCU_OK;
+    for (int i = 0; i < kGraphAttempts && !st; ++i) st = enqueue_attempt(c.get());
+    hipGraph_t gm = nullptr;
+    ce = hipStreamEndCapture(c->stream, &gm);
+    if (st) return st;
+    if (ce != hipSuccess) return fail(RDCU_EDEVICE, std::string("capture: ") + hipGetErrorString(ce));
+    c->graph_m = gm;
+    HIPCHECK(hipGraphInstantiate(&c->gexec_m, gm, nullptr, nullptr, 0));
+  }
   c->nrhs = 0;
   return RDCU_OK;
 }
@@ -1487,7 +1517,7 @@ int rdcu_run(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double
       if (K < 1) K = 1;
     }
     if ((st = begin(c, stop, 0, 0, 0, hmin, hmax, t1))) return st;
-    for (long i = 0; i < K; ++i) HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+    if ((st = launch_attempts(c, K))) return st;
     if ((st = read_state(c))) return st;
     const int nl = c->sh->nlog;
     c->nrhs += 6L * nl;
@@ -1522,7 +1552,7 @@ int rdcu_advance(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, do
   while (k < max_attempts && tc < t1 && hc > hmin) {
     long long K = std::min<long long>(kChunkMax, max_attempts - k);
     if ((st = begin(c, INFINITY, 0, 0, 0, hmin, hmax, t1))) return st;
-    for (long long i = 0; i < K; ++i) HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+    if ((st = launch_attempts(c, K))) return st;
     if ((st = read_state(c))) return st;
     const int nl = c->sh->nlog;
     c->nrhs += 6L * nl;
