@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_rowf_reg(Grid g, FftPlan px, const cx *
 template <int E, bool NT = false, int LS = 0, bool TG = false>
 __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan &p,
                                          const ColGeom &cg, const Bufs &b, const FacList &fl,
-                                         int mode, int fpw) {
+                                         int mode, int fpw, const DevState *chk = nullptr) {
   extern __shared__ cx smem[];
   // mode 1: blocks come in groups of nparts, each transforming up to fpw of
   // the field's factors (the forward transform is repeated per part)
@@ -274,13 +274,16 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   const int tile = lblk / nparts;
   const int grp = tile / cg.ntc, ct = tile - grp * cg.ntc;
   const int g1 = grp / cg.gdiv, g2 = grp - g1 * cg.gdiv;
+  // a part past the field's factors has no work; like an attempt that no
+  // longer runs (chk), it returns after the column loads are issued, so
+  // that their round trip overlaps the loads the decision waits for
   int m0 = 0, m1 = 0;
   if (mode == 1) {
     m0 = fl.fstart[g1] + part * fpw;
     m1 = fl.fstart[g1 + 1];
     if (m1 > m0 + fpw) m1 = m0 + fpw;
-    if (m0 >= m1) return;
   }
+  const bool idle = mode == 1 && m0 >= m1;
   const int L = LS > 0 ? LS : cg.L, T = L / E, tc = cg.tc;
   const int c0 = ct * tc;
   const int nc = (g.Nxh - c0) < tc ? (g.Nxh - c0) : tc;
@@ -316,13 +319,16 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
       nk[2] = kx;
       kv[2] = g.kt[2][kx];
     }
-    blk0 = fl.fblk[m0];
+    blk0 = idle ? 0 : fl.fblk[m0];
   }
+  Prefetch<4, cx> pf_tw;
+  Prefetch<4, double> pf_k;
   if constexpr (!TG) {
-    Prefetch<4, cx> pf_tw;
     pf_tw.load(p.tw, L, tid, blockDim.x);
-    Prefetch<4, double> pf_k;
     if (mode == 1) pf_k.load(g.kt[0], L, tid, blockDim.x);
+  }
+  if (idle || (chk && !attempt_active(chk))) return;
+  if constexpr (!TG) {
     pf_tw.store(twl, p.tw, L, tid, blockDim.x);
     if (mode == 1) pf_k.store(k0l, g.kt[0], L, tid, blockDim.x);
   }
@@ -386,10 +392,9 @@ template <int E, bool NT, int LS = 0, bool TG = false>
 __global__ __launch_bounds__(col_max_threads<E>()) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
                                                  int check_active, Bufs b, FacList fl, int mode,
                                                  int fpw) {
-  if (check_active && !attempt_active(st)) return;
   const int lblk = xcd_block(blockIdx.x, gridDim.x);
   if (lblk >= cg.nblk) return;
-  col_body<E, NT, LS, TG>(lblk, g, p, cg, b, fl, mode, fpw);
+  col_body<E, NT, LS, TG>(lblk, g, p, cg, b, fl, mode, fpw, check_active ? st : nullptr);
 }
 
 // Inverse x pass for one row position r, all fields: factor lines in groups
@@ -476,7 +481,8 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
                                           const FacList &fl, const DictP &dict, int src_kind,
                                           const double *src_ptr, int dst_kind, int dst_slot,
                                           double *dst_ptr, int err, const ErrCoef &ec,
-                                          const StageCoef &nx, int G, cx *smem, int tid, int nt) {
+                                          const StageCoef &nx, int G, cx *smem, int tid, int nt,
+                                          const DevState *chk = nullptr) {
   const int n = GK::n ? GK::n : g.n, Nx = GK::Nx ? GK::Nx : g.Nx;
   const int L = LS > 0 ? LS : g.L, T = L / E;
   const long N = GK::N ? GK::N : g.N, rows = GK::rows ? GK::rows : g.rows;
@@ -516,6 +522,8 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
     const int w = tid + u * nt;
     idr[u] = w < n * Nx ? src[(w / Nx) * N + r * Nx + (w % Nx)] : 0.0;
   }
+  // an attempt that no longer runs (chk) returns here, after the loads
+  if (chk && !attempt_active(chk)) return;
   if constexpr (DE::kProg) pf_p.store(prog, dict.prog, dict.nwords, tid, nt);
 #pragma unroll
   for (int u = 0; u < IDU; ++u) {
@@ -712,7 +720,8 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
                                           const FacList &fl, const DictP &dict, int src_kind,
                                           const double *src_ptr, int dst_kind, int dst_slot,
                                           double *dst_ptr, int err, const ErrCoef &ec,
-                                          const StageCoef &nx, int G) {
+                                          const StageCoef &nx, int G,
+                                          const DevState *chk = nullptr) {
   extern __shared__ cx smem[];
   const int L = LS > 0 ? LS : g.L;
   if constexpr (E <= 8) {
@@ -722,10 +731,11 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
     const int gt = G * (L / E), sub = threadIdx.x / gt;
     rowi_body<E, DE, LS, GK>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl,
                              dict, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
-                             smem + sub * region, threadIdx.x - sub * gt, gt);
+                             smem + sub * region, threadIdx.x - sub * gt, gt, chk);
   } else {
     rowi_body<E, DE, LS, GK>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
-                             dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x);
+                             dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x,
+                             chk);
   }
 }
 
@@ -768,9 +778,8 @@ __global__ __launch_bounds__(256) void k_rowi_reg(Grid g, FftPlan px, const cx *
                                                   const double *src_ptr, int dst_kind, int dst_slot,
                                                   double *dst_ptr, int err, ErrCoef ec,
                                                   StageCoef nx, int G) {
-  if (check_active && !attempt_active(st)) return;
   rowi_rows<E, ProgDict>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot,
-                         dst_ptr, err, ec, nx, G);
+                         dst_ptr, err, ec, nx, G, check_active ? st : nullptr);
 }
 
 // ---------------------------------------------------------------- batches

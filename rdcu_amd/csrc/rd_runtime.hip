@@ -741,7 +741,6 @@ void build_jit(rdcu_ctx *c) {
          "rd::FacList fl, rd::DictP dict, int src_kind, const double *src_ptr, int dst_kind, "
          "int dst_slot, double *dst_ptr, int err, rd::ErrCoef ec, rd::StageCoef nx, int G) {\n";
   const std::string E = std::to_string(c->Ex);
-  src += "  if (check_active && !rd::attempt_active(st)) return;\n";
   // The grid sizes and the factor count are constants of the generated
   // kernel (index arithmetic folds, and the 1-D-only table reads disappear
   // in 2-D and 3-D); the launch passes the same values in g and fl.
@@ -754,7 +753,7 @@ void build_jit(rdcu_ctx *c) {
   src = buf + src;
   src += "  rd::rowi_rows<" + E + ", rd::JitDict, " + std::to_string(c->L) +
          ", rd::JitGeom>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot, "
-         "dst_ptr, err, ec, nx, G);\n}\n";
+         "dst_ptr, err, ec, nx, G, check_active ? st : nullptr);\n}\n";
   const size_t att_lds = attempt1d_lds(c);
   if (att_lds)
     src += "extern \"C\" __global__ __launch_bounds__(256) void rdcu_attempt1d_jit(rd::Grid g, "
