@@ -146,6 +146,30 @@ def test_step_sequence_bitwise(name, Ns, Ls, kind, A, hlib, olib):
     assert nacc > 5
 
 
+@pytest.mark.parametrize("name,Ns,Ls,kind,A", STEP_CASES[:2], ids=[c[0] for c in STEP_CASES[:2]])
+def test_advance_in_groups_bitwise(name, Ns, Ls, kind, A, hlib, olib):
+    """rdcu_advance launches whole groups of attempts through one graph and the
+    rest one by one; counts that are not multiples of the group, and an end
+    time reached inside a group, give the oracle's t, h, attempt count and y."""
+    prob = make_problem(Ns, Ls, kind)
+    y0 = glibc_rand_ic(prob.n, prob.N, seed=1, A=A)
+    print(f"case {name}", flush=True)
+    with Integrator(hlib, prob, y0) as g, Integrator(olib, prob, y0) as o:
+        tg = to = 0.0
+        hg = ho = 0.01
+        for k in (13, 8, 21, 3):
+            tg, hg, dg = g.advance(tg, hg, max_attempts=k)
+            to, ho, do = o.advance(to, ho, max_attempts=k)
+            assert (tg, hg, dg) == (to, ho, do), (k, tg, to, hg, ho, dg, do)
+            assert dg == k
+        t1 = tg + 9.5 * hg  # ends a few attempts into the next group
+        tg, hg, dg = g.advance(tg, hg, t1=t1, max_attempts=64)
+        to, ho, do = o.advance(to, ho, t1=t1, max_attempts=64)
+        assert (tg, hg, dg) == (to, ho, do), (tg, to, hg, ho, dg, do)
+        assert 0 < dg < 64 and tg >= t1
+        assert np.array_equal(g.state(), o.state())
+
+
 def test_run_with_outputs_bitwise(hlib, olib):
     """rdcu_run with output stops (device-side halting, chunked replay) gives
     the same accepted/rejected sequence and dense outputs as the oracle."""
