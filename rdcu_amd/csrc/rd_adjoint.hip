@@ -591,10 +591,11 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
     return RDCU_OK;
   };
   // The step is captured twice, once per role of the two adjoint buffers
-  // (they swap every step), and replayed ns times.
+  // (they swap every step), and replayed ns times; x[2] holds kGraphAttempts
+  // steps (both roles in turn) for the whole groups, which start at even steps.
   struct Execs {
     hipStream_t s;
-    hipGraphExec_t x[2] = {nullptr, nullptr};
+    hipGraphExec_t x[3] = {nullptr, nullptr, nullptr};
     ~Execs() {
       (void)hipStreamSynchronize(s);
       for (auto &e : x)
@@ -602,9 +603,12 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
     }
   } ex{c->stream};
   double *buf[2] = {lam, ybar};
-  for (int par = 0; par < 2 && par < ns; ++par) {
+  for (int par = 0; par < 3 && (par < 2 ? par < ns : ns >= kGraphAttempts); ++par) {
     HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const int s2 = enqueue_step(buf[par], buf[1 - par]);
+    int s2 = RDCU_OK;
+    if (par < 2) s2 = enqueue_step(buf[par], buf[1 - par]);
+    for (int i = 0; par == 2 && i < kGraphAttempts && !s2; ++i)
+      s2 = enqueue_step(buf[i & 1], buf[1 - (i & 1)]);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(c->stream, &g);
     if (s2 || ce != hipSuccess) {
@@ -617,7 +621,10 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
       return fail(RDCU_EDEVICE, std::string("reverse step graph: ") + hipGetErrorString(ie));
   }
   hipLaunchKernelGGL(rd::k_adj_setk, dim3(1), dim3(64), 0, c->stream, kdev, ns - 1);
-  for (long long i = 0; i < ns; ++i) HIPCHECK(hipGraphLaunch(ex.x[i & 1], c->stream));
+  long long i = 0;
+  if (ex.x[2])
+    for (; i + kGraphAttempts <= ns; i += kGraphAttempts) HIPCHECK(hipGraphLaunch(ex.x[2], c->stream));
+  for (; i < ns; ++i) HIPCHECK(hipGraphLaunch(ex.x[i & 1], c->stream));
   if (!kst) c->nrhs += 7 * ns;
   lam = buf[ns & 1];
   std::vector<double> hp((size_t)std::max(1, nt) * c->rows);

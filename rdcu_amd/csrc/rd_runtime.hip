@@ -110,6 +110,7 @@ const double kPI = 3.14159265358979323846;  // the reference's PI (rdcu.cu:14)
 constexpr size_t kLdsMax = 64 * 1024;
 constexpr int kChunkMax = 64;
 constexpr int kGraphAttempts = 8;  // attempts per multi-attempt graph; divides kChunkMax
+static_assert(kGraphAttempts % 2 == 0, "reverse-step groups alternate the two adjoint buffers");
 
 }  // namespace
 
