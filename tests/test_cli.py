@@ -143,3 +143,36 @@ def test_stiff_and_jacobian_output_rejected(tmp_path):
 def test_bad_factor_index_rejected(tmp_path):
     r = run(["-n", 2, "-N", 16, "-c", "1.0,0,99,1", "x"], tmp_path, rc=2)
     assert "out of range" in r.stderr
+
+
+NOTEBOOK_COUPLING = ["1.0,0,0,1", "1.0,0,6,1", "1.0,0,12,1", "-2.000000,0,7,1", "-2.000000,0,13,1",
+                     "-1.0,0,0,3", "-1.0,0,0,1,1,2", "1.0,1,1,1", "1.0,1,7,1", "1.0,1,13,1",
+                     "2.000000,1,6,1", "2.000000,1,12,1", "-1.0,1,0,2,1,1", "-1.0,1,1,3",
+                     "0.800000,1,0,3", "0.800000,1,0,1,1,2", "-0.800000,0,0,2,1,1",
+                     "-0.800000,0,1,3"]
+
+
+def notebook_run(exe, cell, cwd):
+    """Runs a notebook command line (cgle.ipynb; tests/golden/notebook_runs.json)
+    with the cell-3 coupling file (b = 2, c = 0.8) and returns the accepted
+    step count from FILEBASE.out.  The notebook's binary drew its initial
+    condition at amplitude 1 (the help text's "Default 1.0", rdcu.cu:751; the
+    code default is 1e-3, rdcu.cu:603), so -A 1 is added."""
+    run_ = next(r for r in GOLD["runs"] if r["cell"] == cell)
+    args = run_["cmd"].split()[1:]
+    base = args[-1].split("/")[-1]
+    args = args[:-1] + ["-A", "1", base]
+    (cwd / (base + "coupling.dat")).write_text("\n".join(NOTEBOOK_COUPLING) + "\n")
+    run(args, cwd, exe=exe)
+    out = (cwd / (base + ".out")).read_text()
+    return int(out.split("steps: ")[1].split()[0]), run_["accepted"]
+
+
+def test_notebook_256_step_count(tmp_path):
+    """The reference's own run (cgle.ipynb cell 5: 2-D CGLE 256^2, t = 10)
+    accepted 924 steps.  The same command line through the oracle CLI must
+    accept within 1% of that: rounding differences (cuFFT against this FFT)
+    move the count by about one step, a wrong step control or RHS by far more
+    (the code default amplitude 1e-3 gives 856)."""
+    got, ref = notebook_run(ORACLE_CLI, 5, tmp_path)
+    assert abs(got - ref) <= 0.01 * ref, (got, ref)

@@ -163,3 +163,78 @@ def test_fsal_run_equals_repeated_steps(olib):
             log2.append((t, h, ok))
         assert log == log2
         assert np.array_equal(a.state(), b.state())
+
+
+# Dormand-Prince RK5(4)7M tableau as exact fractions, from the method's
+# publication (Dormand & Prince 1980; Hairer, Norsett & Wanner, Table II.5.2):
+# stage matrix, 5th-order weights b and 4th-order weights bs.
+_F = __import__("fractions").Fraction
+_DP_A = [
+    [],
+    [_F(1, 5)],
+    [_F(3, 40), _F(9, 40)],
+    [_F(44, 45), _F(-56, 15), _F(32, 9)],
+    [_F(19372, 6561), _F(-25360, 2187), _F(64448, 6561), _F(-212, 729)],
+    [_F(9017, 3168), _F(-355, 33), _F(46732, 5247), _F(49, 176), _F(-5103, 18656)],
+    [_F(35, 384), _F(0), _F(500, 1113), _F(125, 192), _F(-2187, 6784), _F(11, 84)],
+]
+_DP_BS = [_F(5179, 57600), _F(0), _F(7571, 16695), _F(393, 640), _F(-92097, 339200),
+          _F(187, 2100), _F(1, 40)]
+
+
+def _linear_step_exact(z):
+    """Stage values S_j (Y_j = S_j*y0) of one DP45 step of y' = lam*y with
+    z = lam*h as an exact Fraction, the 5th-order result R = S_7 (FSAL stage)
+    and the embedded error E = z*sum_j (b_j - bs_j) S_j (error = E*y0)."""
+    S = []
+    for s in range(7):
+        S.append(1 + z * sum((a * Sj for a, Sj in zip(_DP_A[s], S)), _F(0)))
+    b = _DP_A[6] + [_F(0)]
+    E = z * sum(((bj - bsj) * Sj for bj, bsj, Sj in zip(b, _DP_BS, S)), _F(0))
+    return S[6], E
+
+
+def test_error_norm_closed_form(olib):
+    """The step-control norm is the scaled RMS of dp45.cu:79-84, 117-119:
+        err_i = h*sum_j e_j k_j,i,  w_i = atol + rtol*max(|y_i|, |ynew_i|),
+        norm = sqrt(sum_i (err_i/w_i)^2) / sqrt(n*N).
+    For y' = lam*y every quantity is a polynomial in z = lam*h times y0, so the
+    norm follows exactly from the tableau (the step may be rejected; the norm
+    is formed either way).  The cases are chosen so that each plausible
+    mistake moves the norm far outside the tolerance in at least one case:
+    growth (z > 0, |ynew| > |y|) catches a max that drops ynew, decay the one
+    that drops y; atol and rtol differ by orders of magnitude at |y| ~ 1
+    (a swap); points of unequal size (dividing by n*N instead of its root);
+    n = 2 fields (normalising by N alone)."""
+    cases = [(3.0, 0.2, [1.0, -0.25], 1e-3, 1e-2, 1),
+             (-4.0, 0.25, [0.75, -2.0], 1e-2, 1e-4, 1),
+             (2.5, 0.3, [1.5, 0.125, -0.5, 2.0], 1e-6, 1e-3, 2)]
+
+    def norm(E, R, y0, atol, rtol, weight, div):
+        tot = sum((E * yi / weight(_F(atol), _F(rtol), abs(yi), abs(R * yi))) ** 2
+                  for yi in map(_F, y0))
+        return float(tot / div) ** 0.5
+
+    right = lambda a, r, y, yn: a + r * max(y, yn)
+    mistakes = {"max drops ynew": (lambda a, r, y, yn: a + r * y, None),
+                "max drops y": (lambda a, r, y, yn: a + r * yn, None),
+                "atol/rtol swapped": (lambda a, r, y, yn: r + a * max(y, yn), None),
+                "divided by (nN)^2": (right, "sq"),
+                "divided by N only": (right, "N")}
+    caught = set()
+    for lam, h, y0, atol, rtol, n in cases:
+        terms = [Term(lam, f, [(f, 1)]) for f in range(n)]
+        prob = scalar_problem(terms, n=n)
+        with Integrator(olib, prob, np.array(y0), atol=atol, rtol=rtol) as it:
+            it.step(0.0, h)
+            got = olib.oracle_last_norm(it.h)
+        R, E = _linear_step_exact(_F(lam) * _F(h))
+        want = norm(E, R, y0, atol, rtol, right, len(y0))
+        # the computed error sum cancels O(z) stage terms down to O(z^5):
+        # rounding leaves about 1e-13 relative at these z
+        assert abs(got - want) <= 1e-11 * want, (lam, h, got, want)
+        for name, (wfun, div) in mistakes.items():
+            d = {None: len(y0), "sq": len(y0) ** 2, "N": len(y0) // n}[div]
+            if abs(norm(E, R, y0, atol, rtol, wfun, d) - want) > 1e-6 * want:
+                caught.add(name)
+    assert caught == set(mistakes), set(mistakes) - caught
