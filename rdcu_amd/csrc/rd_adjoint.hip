@@ -240,7 +240,7 @@ int vjp(rdcu_ctx *c, const double *u, const double *w, double *out_id, double *G
   rd::StageCoef none{};
   if (!pre) {
     rowf(c, 0, rd::SRC_PTR, u, none, c->rf_rpb);
-    for (auto &l : col_launches_reg(c, 0, c->cr_tc)) l.go();
+    for (auto &l : col_launches_reg(c, 0, c->cr_tc, c->cr_ntc)) l.go();
   }
   const rd::Bufs bv = pre ? *pre : c->bufs;
   const int nf = c->nf_used, T = c->L / c->Ex;
@@ -530,7 +530,7 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
     c->bufs.F = F7;
     rd::StageCoef none{};
     rowf(c, 0, rd::SRC_PTR, Yall, none, c->rf_rpb);
-    for (auto &l : col_launches_reg(c, 0, c->cr_tc)) l.go();
+    for (auto &l : col_launches_reg(c, 0, c->cr_tc, c->cr_ntc)) l.go();
     c->grid = g0;
     c->bufs = b0;
     c->fl_used = f0;

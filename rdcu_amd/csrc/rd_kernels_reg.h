@@ -285,8 +285,13 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   }
   const bool idle = mode == 1 && m0 >= m1;
   const int L = LS > 0 ? LS : cg.L, T = L / E, tc = cg.tc;
-  const int c0 = ct * tc;
-  const int nc = (g.Nxh - c0) < tc ? (g.Nxh - c0) : tc;
+  // the Nxh columns of a group are split into ntc tiles whose widths differ
+  // by at most one (tc >= the widest), so that a launch can be sized to the
+  // CU count: at 512^2 the 257 columns of a field fill 64 tiles of 4 or 5
+  // instead of 64 tiles of 4 plus a 65th of one column, which put a second
+  // block on one CU of every XCD
+  const int c0 = (int)((long)ct * g.Nxh / cg.ntc);
+  const int nc = (int)((long)(ct + 1) * g.Nxh / cg.ntc) - c0;
   const int tid = threadIdx.x;
   const int c = tid % tc, t = tid / tc;
   const bool act = c < nc;

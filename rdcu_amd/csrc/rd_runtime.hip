@@ -165,6 +165,7 @@ struct rdcu_ctx {
   // strided axes (0 = not eligible); shapes picked by the autotuner
   int reg = 0, Ex = 0, E0 = 0, E1 = 0;
   int rf_rpb = 1, cr_tc = 1, ri_G = 1, ri_R = 1, cr_fpw = 1, max_fpf = 1;
+  int cr_ntc = 0;  // strided tiles per column group (0: ceil(Nxh / cr_tc))
   size_t lds_rf = 0, lds_cr0 = 0, lds_cr1 = 0, lds_ri = 0;
   bool lds_rhs = true, lds_y = true;  // the LDS kernels fit (see choose_shapes)
   std::string tune_log;
@@ -500,8 +501,16 @@ rd::StageCoef stage_coef(int stage) {
   return sc;
 }
 
+// Column tiles of the strided passes: ntc tiles per group (0: as many
+// tiles of tc columns as the Nxh columns need), each block as wide as the
+// widest tile (the kernel splits the columns evenly over the tiles).
+void col_tiles(const rdcu_ctx *c, int &tc, int &ntc) {
+  if (ntc <= 0 || ntc > c->Nxh) ntc = (c->Nxh + tc - 1) / tc;
+  tc = (c->Nxh + ntc - 1) / ntc;
+}
+
 // Strided passes of the register path for one RHS (after the x forward pass).
-std::vector<Launch> col_launches_reg(rdcu_ctx *c, int check, int tc) {
+std::vector<Launch> col_launches_reg(rdcu_ctx *c, int check, int tc, int ntc) {
   std::vector<Launch> v;
   if (c->col_lds) {  // LDS column kernels inside the register path (comparison switch)
     for (auto &l : rhs_launches(c, 0, rd::SRC_Y, nullptr, rd::DST_PTR, 0, nullptr, check, 0))
@@ -509,7 +518,7 @@ std::vector<Launch> col_launches_reg(rdcu_ctx *c, int check, int tc) {
     return v;
   }
   const int nd = c->ndim;
-  const int ntc = (c->Nxh + tc - 1) / tc;
+  col_tiles(c, tc, ntc);
   if (nd == 3) {
     rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc, tc, 1};
     long ng = (long)c->n * c->Ns[0];
@@ -546,7 +555,7 @@ std::vector<Launch> rhs_launches_reg(rdcu_ctx *c, int src_kind, const double *sr
   std::vector<Launch> v;
   rd::StageCoef none{};
   v.push_back({"row_fwd", [=]() { rowf(c, check, src_kind, src_ptr, none, c->rf_rpb); }});
-  for (auto &l : col_launches_reg(c, check, c->cr_tc)) v.push_back(l);
+  for (auto &l : col_launches_reg(c, check, c->cr_tc, c->cr_ntc)) v.push_back(l);
   v.push_back({"row_inv", [=]() {
                  rowi(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, 0, none, c->ri_G, c->ri_R);
                }});
@@ -561,7 +570,7 @@ std::vector<Launch> attempt_launches_reg(rdcu_ctx *c) {
   rd::StageCoef s2 = stage_coef(2);
   v.push_back({"row_fwd", [=]() { rowf(c, 1, rd::SRC_STAGE, nullptr, s2, c->rf_rpb); }});
   for (int s = 2; s <= 7; ++s) {
-    for (auto &l : col_launches_reg(c, 1, c->cr_tc)) v.push_back(l);
+    for (auto &l : col_launches_reg(c, 1, c->cr_tc, c->cr_ntc)) v.push_back(l);
     rd::StageCoef nx = s < 7 ? stage_coef(s + 1) : rd::StageCoef{};
     int dk = s < 7 ? rd::DST_KS : rd::DST_K7;
     int err = s == 7 && !c->fixed;
@@ -810,9 +819,10 @@ int pow2_above(int x) {
 }
 
 int fixed_shapes(rdcu_ctx *c, const char *spec) {
-  int rpb = 0, tc = 0, fpw = 0, G = 0, R = 1, Ec = 0;
-  const int got = sscanf(spec, "%d,%d,%d,%d,%d,%d", &rpb, &tc, &fpw, &G, &R, &Ec);
-  if (got < 4) return fail(RDCU_EINVAL, "RDCU_SHAPES must be rpb,tc,fpw,G[,R[,Ecol]]");
+  int rpb = 0, tc = 0, fpw = 0, G = 0, R = 1, Ec = 0, ntc = 0;
+  const int got = sscanf(spec, "%d,%d,%d,%d,%d,%d,%d", &rpb, &tc, &fpw, &G, &R, &Ec, &ntc);
+  if (got < 4) return fail(RDCU_EINVAL, "RDCU_SHAPES must be rpb,tc,fpw,G[,R[,Ecol[,ntc]]]");
+  if (c->ndim >= 2 && tc >= 1) col_tiles(c, tc, ntc);
   if (Ec == 16) {  // 16 values per thread on the power-of-two strided axes
     if (c->E0 == 8 && c->Ns[0] >= 16) c->E0 = 16;
     if (c->ndim == 3 && c->E1 == 8 && c->Ns[1] >= 16) c->E1 = 16;
@@ -837,6 +847,7 @@ int fixed_shapes(rdcu_ctx *c, const char *spec) {
   if (!ok) return fail(RDCU_EINVAL, std::string("RDCU_SHAPES out of range: ") + spec);
   c->rf_rpb = rpb;
   c->cr_tc = tc;
+  c->cr_ntc = ntc;
   c->cr_fpw = fpw;
   c->ri_G = G;
   c->ri_R = R;
@@ -894,25 +905,35 @@ int autotune(rdcu_ctx *c) {
         Lmax = std::max(Lmax, c->Ns[1]);
         Tmax = std::max(Tmax, c->Ns[1] / c->E1);
       }
-      for (int tc = std::max(1, 64 / Tmax); tc <= 64; tc *= 2) {
-        if (tc * Tmax > col_threads(c) ||
-            !col_fits(c, tc, 1))
-          break;
-        for (int fpw = c->max_fpf; fpw >= 1; fpw = fpw > 1 ? (fpw + 1) / 2 : 0) {
-          if (!col_fits(c, tc, fpw)) continue;
-          c->cr_fpw = fpw;
-          auto ls = col_launches_reg(c, 0, tc);
-          double tm = time_of([&]() { for (auto &l : ls) l.go(); });
-          snprintf(line, sizeof line, "col E=%d tc=%d fpw=%d %.2fus; ", c->E0, tc, fpw, tm * 1e3);
-          c->tune_log += line;
-          if (tm < best) {
-            best = tm;
-            c->cr_tc = tc;
-            best_fpw = fpw;
-            bestE0 = c->E0;
-            bestE1 = c->E1;
+      for (int tc0 = std::max(1, 64 / Tmax); tc0 <= 64; tc0 *= 2) {
+        if (tc0 * Tmax > col_threads(c) || !col_fits(c, tc0, 1)) break;
+        // tiles of tc0 columns (the last one holding the remainder), and the
+        // same columns in one tile fewer, spread evenly (blocks one column
+        // wider), so that the block count can land on a multiple of the CUs
+        int ntcs[2] = {(c->Nxh + tc0 - 1) / tc0, c->Nxh / tc0};
+        for (int k = 0; k < 2; ++k) {
+          int tc = tc0, ntc = ntcs[k];
+          if (ntc < 1 || (k == 1 && ntc == ntcs[0])) continue;
+          col_tiles(c, tc, ntc);
+          if (tc * Tmax > col_threads(c) || !col_fits(c, tc, 1)) continue;
+          for (int fpw = c->max_fpf; fpw >= 1; fpw = fpw > 1 ? (fpw + 1) / 2 : 0) {
+            if (!col_fits(c, tc, fpw)) continue;
+            c->cr_fpw = fpw;
+            auto ls = col_launches_reg(c, 0, tc, ntc);
+            double tm = time_of([&]() { for (auto &l : ls) l.go(); });
+            snprintf(line, sizeof line, "col E=%d tc=%d ntc=%d fpw=%d %.2fus; ", c->E0, tc, ntc,
+                     fpw, tm * 1e3);
+            c->tune_log += line;
+            if (tm < best) {
+              best = tm;
+              c->cr_tc = tc;
+              c->cr_ntc = ntc;
+              best_fpw = fpw;
+              bestE0 = c->E0;
+              bestE1 = c->E1;
+            }
+            if (fpw == 1) break;
           }
-          if (fpw == 1) break;
         }
       }
     }
@@ -1442,9 +1463,9 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
     }
     const char *vb = getenv("RDCU_VERBOSE");
     if (vb && *vb && *vb != '0')
-      fprintf(stderr, "rdcu jit: %s\nrdcu autotune: %s\nrdcu shapes: %d,%d,%d,%d,%d,%d\n",
+      fprintf(stderr, "rdcu jit: %s\nrdcu autotune: %s\nrdcu shapes: %d,%d,%d,%d,%d,%d,%d\n",
               c->jit_log.c_str(), c->tune_log.c_str(), c->rf_rpb, c->cr_tc, c->cr_fpw, c->ri_G,
-              c->ri_R, c->E0 == 16 ? 16 : 8);
+              c->ri_R, c->E0 == 16 ? 16 : 8, c->cr_ntc);
   }
 
   // capture one attempt
