@@ -125,6 +125,19 @@ int rdcu_set_state(rdcu_ctx *ctx, const double *y);
  * uninitialised memory in block 0). */
 int rdcu_rhs(rdcu_ctx *ctx, const double *y, double *f_out, double *Y_out);
 
+/* rdcu_rhs that allocates nothing: the same values, bit for bit, with every
+ * intermediate device buffer of the call (input staging, x-forward spectra,
+ * factor spectra, output staging, factor blocks) taken from one static device
+ * array of 128 MiB per GPU, which each GPU the library runs on holds until the
+ * process ends.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory during their loop (tight serving or fitting loops,
+ * harnesses that forbid hipMalloc); the context itself is created beforehand.
+ * A problem needing more scratch than the array holds is rejected with
+ * RDCU_EINVAL.  Calls on one GPU share its array: within a process they are
+ * serialised (concurrent callers wait); nothing in it needs to start at zero,
+ * and it keeps the last call's contents. */
+int rdcu_rhs_noalloc(rdcu_ctx *ctx, const double *y, double *f_out, double *Y_out);
+
 /* One output frame of step_eval (rdcu.cu:252-341): the dense output at t_eval
  * into y_out[n*N] (as rdcu_eval) and, when not NULL, the RHS at that state
  * into f_out and its factor blocks into Y_out (as rdcu_rhs on y_out).  Same

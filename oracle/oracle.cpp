@@ -1322,6 +1322,11 @@ int rdcu_rhs(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
 }
 
 // One output frame: the dense output, then the RHS at it (rdcu.cu:270-291).
+// The CPU build has no device scratch: the same values as rdcu_rhs.
+int rdcu_rhs_noalloc(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
+  return rdcu_rhs(c, y, f_out, Y_out);
+}
+
 int rdcu_frame(rdcu_ctx *c, double t_eval, double *y_out, double *f_out, double *Y_out) {
   if (!c || !y_out) return fail(RDCU_EINVAL, "null argument");
   int st = rdcu_eval(c, t_eval, y_out);

@@ -34,6 +34,7 @@ SYMBOLS = (
     "rdcu_batch_create", "rdcu_batch_destroy", "rdcu_batch_step", "rdcu_batch_run",
     "rdcu_batch_member", "rdcu_set_coef", "rdcu_schedule", "rdcu_loss_grad",
     "rdcu_loss_grad_batch", "rdcu_frame", "rdcu_host_alloc", "rdcu_host_free",
+    "rdcu_rhs_noalloc",
 )
 
 
@@ -93,6 +94,7 @@ def load(path: str) -> ctypes.CDLL:
     lib.rdcu_get_state.argtypes = [vp, _DP]
     lib.rdcu_set_state.argtypes = [vp, _DP]
     lib.rdcu_rhs.argtypes = [vp, _DP, _DP, _DP]
+    lib.rdcu_rhs_noalloc.argtypes = [vp, _DP, _DP, _DP]
     lib.rdcu_frame.argtypes = [vp, ctypes.c_double, _DP, _DP, _DP]
     lib.rdcu_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
     lib.rdcu_host_free.argtypes = [vp]
@@ -301,14 +303,15 @@ class Integrator:
         y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
         self._check(self.lib.rdcu_set_state(self.h, _ptr(y)))
 
-    def rhs(self, y: np.ndarray, factors=False):
+    def rhs(self, y: np.ndarray, factors=False, noalloc=False):
         """f = dydt(y); with factors=True also every Y block as complex
-        [nblk, n, N] in the layout of FILEBASEY.dat."""
+        [nblk, n, N] in the layout of FILEBASEY.dat.  noalloc=True calls
+        rdcu_rhs_noalloc (the same values; no device allocation)."""
         y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
         f = np.empty(self.nN)
         Y = np.empty(2 * self.nN * self.prob.nblk) if factors else None
-        self._check(self.lib.rdcu_rhs(self.h, _ptr(y), _ptr(f),
-                                      _ptr(Y) if factors else None))
+        fn = self.lib.rdcu_rhs_noalloc if noalloc else self.lib.rdcu_rhs
+        self._check(fn(self.h, _ptr(y), _ptr(f), _ptr(Y) if factors else None))
         if factors:
             return f, Y.view(np.complex128).reshape(self.prob.nblk, self.prob.n, self.prob.N)
         return f

@@ -112,6 +112,37 @@ constexpr int kChunkMax = 64;
 constexpr int kGraphAttempts = 8;  // attempts per multi-attempt graph; divides kChunkMax
 static_assert(kGraphAttempts % 2 == 0, "reverse-step groups alternate the two adjoint buffers");
 
+// Device scratch of rdcu_rhs_noalloc: one array per GPU for the life of the
+// process, so that the call allocates nothing.  Host code never uses the
+// array's name as an address (on the host it is only a stand-in); the
+// device address comes from hipGetSymbolAddress, once per device.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+__device__ __align__(256) char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_call_mu[kMaxDevices];
+
+size_t round_up256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Device address of this GPU's scratch array (the device must be current).
+int scratch_base(int dev, char **out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(RDCU_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(RDCU_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<char *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return RDCU_OK;
+}
+
+// Calls on one GPU share its array: they are serialised within the process.
+std::mutex &scratch_mutex(int dev) { return g_scratch_call_mu[dev]; }
+
 }  // namespace
 
 struct rdcu_ctx {
@@ -1280,6 +1311,10 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
   int st = RDCU_OK;
   HIPCHECK(hipSetDevice(device));
   c->dev = device;
+  {  // set-up of this GPU's rdcu_rhs_noalloc scratch address (once per device)
+    char *base = nullptr;
+    if ((st = scratch_base(device, &base))) return st;
+  }
   if (mo) c->stream = mo->stream;
   else HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 
@@ -1692,6 +1727,60 @@ int rdcu_rhs(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
   const long nN = (long)c->n * c->N;
   HIPCHECK(hipMemcpyAsync(c->rhs_in, y, nN * sizeof(double), hipMemcpyHostToDevice, c->stream));
   int st = enqueue_rhs_out(c, c->rhs_in, f_out, Y_out);
+  if (st) return st;
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return RDCU_OK;
+}
+
+int rdcu_rhs_noalloc(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
+  if (!c || !y) return fail(RDCU_EINVAL, "null argument");
+  if (Y_out && !c->lds_y) return fail(RDCU_EUNSUPPORTED, "factor output not available for lines this long");
+  HIPCHECK(hipSetDevice(c->dev));
+  char *base = nullptr;
+  int st = scratch_base(c->dev, &base);
+  if (st) return st;
+  // every intermediate buffer of rdcu_rhs, carved out of the scratch array
+  const long nN = (long)c->n * c->N;
+  const int nfac = Y_out ? c->nf_all : c->nf_used;
+  const size_t b_in = round_up256(nN * sizeof(double));
+  const size_t b_S = round_up256((size_t)c->n * c->Nh * sizeof(cx));
+  const size_t b_F = round_up256(c->ndim >= 2 ? (size_t)std::max(1, nfac) * c->Nh * sizeof(cx) : 16);
+  const size_t b_out = b_in;
+  const size_t b_Yr = Y_out ? round_up256((size_t)std::max(1, c->nf_all) * c->N * sizeof(double)) : 0;
+  const size_t b_Yc = Y_out ? round_up256(2 * (size_t)c->nblk * nN * sizeof(double)) : 0;
+  const size_t need = b_in + b_S + b_F + b_out + b_Yr + b_Yc;
+  if (need > kScratchBytes)
+    return fail(RDCU_EINVAL, "rdcu_rhs_noalloc: this problem needs " + std::to_string(need) +
+                                 " bytes of device scratch, more than the " +
+                                 std::to_string(kScratchBytes) + " the library holds per GPU");
+  std::lock_guard<std::mutex> lk(scratch_mutex(c->dev));
+  // the context's own buffers are swapped out for the call, then restored
+  struct Swap {
+    rdcu_ctx *c;
+    rd::Bufs b;
+    double *in, *out, *Yr, *Yc;
+    ~Swap() {
+      c->bufs = b;
+      c->rhs_in = in;
+      c->rhs_out = out;
+      c->Yreal = Yr;
+      c->Ycx = Yc;
+    }
+  } keep{c, c->bufs, c->rhs_in, c->rhs_out, c->Yreal, c->Ycx};
+  char *p = base;
+  c->rhs_in = reinterpret_cast<double *>(p);
+  p += b_in;
+  c->bufs.S = reinterpret_cast<cx *>(p);
+  p += b_S;
+  c->bufs.F = reinterpret_cast<cx *>(p);
+  p += b_F;
+  c->rhs_out = reinterpret_cast<double *>(p);
+  p += b_out;
+  c->Yreal = Y_out ? reinterpret_cast<double *>(p) : nullptr;
+  p += b_Yr;
+  c->Ycx = Y_out ? reinterpret_cast<double *>(p) : nullptr;
+  HIPCHECK(hipMemcpyAsync(c->rhs_in, y, nN * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  st = enqueue_rhs_out(c, c->rhs_in, f_out, Y_out);
   if (st) return st;
   HIPCHECK(hipStreamSynchronize(c->stream));
   return RDCU_OK;
