@@ -120,9 +120,10 @@ int rdcu_set_state(rdcu_ctx *ctx, const double *y);
 /* Evaluates the dictionary RHS at y[n*N]: f_out[n*N] (may be NULL) and, if
  * Y_out is not NULL, all n*(1+ndim+ndim^2) factor blocks as complex values
  * Y_out[2*((blk*n+field)*N + x) + {0,1}] in the layout of FILEBASEY.dat.
- * Replaces dydt/makeY (rdcu.cu:117-184).  Imaginary parts are written as 0
- * (the reference leaves the anti-Hermitian Nyquist leakage there, and
- * uninitialised memory in block 0). */
+ * Replaces dydt/makeY (rdcu.cu:117-184).  The imaginary parts are those of
+ * the reference's complex transform (rdcu.cu:151-164): the Nyquist leakage of
+ * the first and mixed derivative blocks; 0 in block 0 (uninitialised memory
+ * in the reference) and in the pure second derivatives (roundoff there). */
 int rdcu_rhs(rdcu_ctx *ctx, const double *y, double *f_out, double *Y_out);
 
 /* rdcu_rhs that allocates nothing: the same values, bit for bit, with every

@@ -35,8 +35,10 @@
 // the observed 5th order, Decimal arithmetic for pow(x,-0.2), the glibc rand()
 // initial values, and one test per step-control branch.
 //
-// Deviation, stated: FILEBASEY.dat imaginary parts are written as 0 (the
-// reference writes Nyquist leakage plus uninitialised memory there).
+// FILEBASEY.dat imaginary parts: the Nyquist leakage of the first and mixed
+// derivative blocks (the transform of the anti-Hermitian part of the
+// reference's multiplier); 0 in block 0 (uninitialised memory in the
+// reference) and in the pure second derivatives (roundoff there).
 
 #include <cmath>
 #include <cstdint>
@@ -451,9 +453,29 @@ inline double ipow(double v, int e) {
 
 // Hermitian spectral multiplier of derivative block blk at wavenumber index
 // nk[0..ndim-1] (last axis in [0, Nx/2]).
+// Codes blk >= kImagBlk: -i times the anti-Hermitian part of the reference's
+// multiplier of block blk - kImagBlk, whose transform is the imaginary part
+// the reference's complex makeY leaves in Y (rdcu.cu:151-164): i*k_a on the
+// Nyquist plane of axis a for first derivatives, -k_a*k_b where exactly one of
+// a, b is at Nyquist for mixed ones, zero elsewhere.
+constexpr int kImagBlk = 64;
 inline cx apply_mult(const Problem &P, int blk, const int *nk, cx X) {
   const int nd = P.ndim;
   auto nyq = [&](int a) { return nk[a] == P.Ns[a] / 2; };
+  if (blk >= kImagBlk) {
+    blk -= kImagBlk;
+    if (blk <= nd) {
+      int a = blk - 1;
+      if (!nyq(a)) return {0.0, 0.0};
+      double val = P.kt[a][nk[a]] * P.invN;
+      return {X.re * val, X.im * val};
+    }
+    int qd = blk - 1 - nd;
+    int a = qd / nd, b = qd % nd;
+    if (a == b || nyq(a) == nyq(b)) return {0.0, 0.0};
+    double val = (P.kt[a][nk[a]] * P.kt[b][nk[b]]) * P.invN;
+    return {-(X.im * val), X.re * val};
+  }
   if (blk <= nd) {
     int a = blk - 1;
     if (nyq(a)) return {0.0, 0.0};
@@ -1317,16 +1339,34 @@ int rdcu_rhs(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
           dst[2 * x + 1] = 0.0;
         }
       }
+    // imaginary parts: the kImagBlk transforms of the blocks that leak
+    // (first and mixed derivatives); block 0 and pure second derivatives
+    // stay 0 (the reference leaves roundoff, and in block 0 uninitialised
+    // memory, there)
+    std::vector<Factor> im = c->E.all;
+    for (Factor &fc : im) fc.blk += kImagBlk;
+    c->E.factors(y, im);
+    for (int blk = 1; blk < P.nblk; ++blk) {
+      const int q = blk - 1 - P.ndim;
+      if (blk > P.ndim && q / P.ndim == q % P.ndim) continue;
+      for (int fld = 0; fld < P.n; ++fld) {
+        double *dst = Y_out + 2 * ((size_t)(blk * P.n + fld) * N);
+        int m = 0;
+        while (!(im[m].field == fld && im[m].blk == blk + kImagBlk)) ++m;
+        const double *src = &c->E.R[(size_t)m * N];
+        for (long x = 0; x < N; ++x) dst[2 * x + 1] = src[x];
+      }
+    }
   }
   return RDCU_OK;
 }
 
-// One output frame: the dense output, then the RHS at it (rdcu.cu:270-291).
 // The CPU build has no device scratch: the same values as rdcu_rhs.
 int rdcu_rhs_noalloc(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
   return rdcu_rhs(c, y, f_out, Y_out);
 }
 
+// One output frame: the dense output, then the RHS at it (rdcu.cu:270-291).
 int rdcu_frame(rdcu_ctx *c, double t_eval, double *y_out, double *f_out, double *Y_out) {
   if (!c || !y_out) return fail(RDCU_EINVAL, "null argument");
   int st = rdcu_eval(c, t_eval, y_out);

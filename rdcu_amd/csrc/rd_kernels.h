@@ -25,6 +25,9 @@ namespace rd {
 constexpr int kNMax = 8;       // fields per problem handled by the kernels
 constexpr int kLogCap = 128;   // attempts logged per chunk
 constexpr int kRedCap = 4096;  // lines reduced in LDS by one block
+// Factor codes at and above kImagBlk select the imaginary part of derivative
+// block (code - kImagBlk) in Y (apply_mult); real blocks are below 1+3+9.
+constexpr int kImagBlk = 64;
 
 // Device-resident integrator state (replaces dp45.cu's static globals and the
 // host-side step control).
@@ -129,6 +132,35 @@ __device__ __forceinline__ cx apply_mult(const Grid &g, int blk, const int *nk, 
     val = (g.kt[a][nk[a]] * g.kt[b][nk[b]]) * g.invN;
   }
   return {-(X.re * val), -(X.im * val)};
+}
+
+// apply_mult, and for codes blk >= kImagBlk the imaginary part of Y (the
+// factor output of the LDS kernels; the integration path never sees these
+// codes and calls apply_mult, which keeps its registers).
+__device__ __forceinline__ cx apply_mult_y(const Grid &g, int blk, const int *nk, cx X) {
+  const int nd = g.ndim;
+  if (blk >= kImagBlk) {
+    // -i times the anti-Hermitian part of the reference's multiplier m
+    // (rdcu.cu:89-95): (m(k) - conj(m(-k)))/2 is m itself where m(-k) is
+    // not conj(m(k)), i.e. i*k_a on the Nyquist plane of axis a, and
+    // -k_a*k_b where exactly one of a, b is at Nyquist; zero elsewhere.  Its
+    // transform is the imaginary part the reference leaves in Y.
+    blk -= kImagBlk;
+    if (blk <= nd) {
+      const int a = blk - 1;
+      if (nk[a] != g.Ns[a] / 2) return {0.0, 0.0};
+      const double val = g.kt[a][nk[a]] * g.invN;
+      return {X.re * val, X.im * val};
+    }
+    const int q = blk - 1 - nd;
+    const int a = q / nd, b = q - a * nd;
+    if (a == b) return {0.0, 0.0};
+    const bool na = nk[a] == g.Ns[a] / 2, nb = nk[b] == g.Ns[b] / 2;
+    if (na == nb) return {0.0, 0.0};
+    const double val = (g.kt[a][nk[a]] * g.kt[b][nk[b]]) * g.invN;
+    return {-(X.im * val), X.re * val};
+  }
+  return apply_mult(g, blk, nk, X);
 }
 
 __device__ __forceinline__ double ipow(double v, int e) {
@@ -354,7 +386,7 @@ __global__ void k_col(Grid g, FftPlan p, ColGeom cg, DevState *st, int check_act
         nk[1] = g2;
         nk[2] = c0 + c;
       }
-      Cb[e * nc + c] = apply_mult(g, blk, nk, R[e * nc + c]);
+      Cb[e * nc + c] = apply_mult_y(g, blk, nk, R[e * nc + c]);
     }
     __syncthreads();
     const cx *Q = lds_fft<+1, true>(Cb, C, p, nc, 1, nc, tid, nt);
@@ -411,8 +443,8 @@ __global__ void k_row_c2r(Grid g, FftPlan px, const cx *__restrict__ twN, DevSta
         const cx *row = b.S + (long)fl.ffield[m] * g.Nh;
         const int blk = fl.fblk[m];
         int nk0 = k, nk1 = L - k;
-        Av = apply_mult(g, blk, &nk0, row[k]);
-        Bv = apply_mult(g, blk, &nk1, row[L - k]);
+        Av = apply_mult_y(g, blk, &nk0, row[k]);
+        Bv = apply_mult_y(g, blk, &nk1, row[L - k]);
       } else {
         const cx *row = b.F + (long)m * g.Nh + r * g.Nxp;
         Av = row[k];
@@ -739,6 +771,27 @@ __global__ void k_pack_y(const double *__restrict__ y, const double *__restrict_
   const double v =
       blk == 0 ? y[(long)f * N + x] : Yr[((long)f * (nblk - 1) + blk - 1) * N + x];
   out[(long)bf * N + x] = make_double2(v, 0.0);
+}
+
+// Whether derivative block blk has an imaginary part in Y: first
+// derivatives and mixed second derivatives (Nyquist leakage); block 0 and
+// the pure second derivatives have none.
+__host__ __device__ inline bool y_has_imag(int blk, int nd) {
+  if (blk < 1) return false;
+  if (blk <= nd) return true;
+  const int q = blk - 1 - nd;
+  return q / nd != q % nd;
+}
+
+// Imaginary parts of Y from the transforms of the kImagBlk factors (Yi, the
+// layout of k_pack_y's Yr): written into out's .y of the blocks that have one.
+__global__ void k_pack_y_imag(const double *__restrict__ Yi, double2 *__restrict__ out, int n,
+                              int nblk, int nd, long N) {
+  const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  const int bf = blockIdx.y, blk = bf / n, f = bf - blk * n;
+  if (!y_has_imag(blk, nd)) return;
+  out[(long)bf * N + x].y = Yi[((long)f * (nblk - 1) + blk - 1) * N + x];
 }
 
 }  // namespace rd

@@ -175,6 +175,7 @@ struct rdcu_ctx {
   std::vector<double> h_tcoef;
   int col_lds = 0;
   rd::FacList fl_used{}, fl_all{};
+  rd::FacList fl_imag{};  // fl_all with the imaginary-part codes (Y output)
   rd::DevState *st = nullptr;
   rd::DevState *sh = nullptr;  // pinned host shadow
   double *rhs_in = nullptr, *rhs_out = nullptr, *Yreal = nullptr, *yeval = nullptr;
@@ -1032,9 +1033,10 @@ int allow_big_lds() {
   return RDCU_OK;
 }
 
-// Every derivative block of every field, real space, into c->Yreal
-// (S must hold the current forward spectra).
-int enqueue_all_factors(rdcu_ctx *c) {
+// Every factor of fl (every derivative block of every field: fl_all, or
+// their imaginary parts: fl_imag), real space, into out (S must hold the
+// current forward spectra; it is only read).
+int enqueue_all_factors(rdcu_ctx *c, const rd::FacList &fl, double *out) {
   if (c->nf_all == 0) return RDCU_OK;
   const int nd = c->ndim;
   const int ntc = (c->Nxh + c->col_tc - 1) / c->col_tc;
@@ -1054,20 +1056,20 @@ int enqueue_all_factors(rdcu_ctx *c) {
     }
     dim3 gg((unsigned)((long)c->n * cg.gdiv * ntc));
     hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream, c->grid, c->p0, cg,
-                       c->st, 0, c->bufs, c->fl_all, 1);
+                       c->st, 0, c->bufs, fl, 1);
   }
   if (nd == 3) {
     rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc,
                    c->col_tc, 1};
     dim3 gg((unsigned)((long)c->nf_all * c->Ns[0] * ntc));
     hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream, c->grid, c->p1, cg,
-                       c->st, 0, c->bufs, c->fl_all, 2);
+                       c->st, 0, c->bufs, fl, 2);
   }
   rd::ErrCoef ec{};
   hipLaunchKernelGGL(rd::k_row_c2r, dim3((unsigned)c->rows), dim3(c->c2r_thr), c->lds_c2r_all,
-                     c->stream, c->grid, c->px, c->twN, c->st, 0, c->bufs, c->fl_all, c->dict, 1,
+                     c->stream, c->grid, c->px, c->twN, c->st, 0, c->bufs, fl, c->dict, 1,
                      (int)rd::SRC_PTR, (const double *)nullptr, (int)rd::DST_PTR, 0,
-                     (double *)nullptr, 0, ec, c->c2r_grp_all, c->Yreal);
+                     (double *)nullptr, 0, ec, c->c2r_grp_all, out);
   HIPCHECK(hipGetLastError());
   return RDCU_OK;
 }
@@ -1401,6 +1403,11 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
     c->h_ufld = ufld;
     c->h_prog = prog;
     c->fl_all = rd::FacList{c->nf_all, ab, af, as};
+    std::vector<int> iblk(ablk);
+    for (int &v : iblk) v += rd::kImagBlk;
+    int *ib;
+    if ((st = c->upload(&ib, iblk))) return st;
+    c->fl_imag = rd::FacList{c->nf_all, ib, af, as};
   }
   if ((st = choose_shapes(c.get()))) return st;
   // register-resident path when every transform length allows it
@@ -1710,10 +1717,16 @@ int enqueue_rhs_out(rdcu_ctx *c, const double *y_dev, double *f_out, double *Y_o
     if (!c->Yreal && (st = c->dalloc(&c->Yreal, (size_t)std::max(1, c->nf_all) * N))) return st;
     const size_t ny = 2 * (size_t)c->nblk * nN;
     if (!c->Ycx && (st = c->dalloc(&c->Ycx, ny))) return st;
-    if ((st = enqueue_all_factors(c))) return st;
-    hipLaunchKernelGGL(rd::k_pack_y, dim3((unsigned)((N + 255) / 256), (unsigned)(c->nblk * c->n)),
-                       dim3(256), 0, c->stream, y_dev, c->Yreal, (double2 *)c->Ycx, c->n, c->nblk,
-                       N);
+    const dim3 pg((unsigned)((N + 255) / 256), (unsigned)(c->nblk * c->n));
+    if ((st = enqueue_all_factors(c, c->fl_all, c->Yreal))) return st;
+    hipLaunchKernelGGL(rd::k_pack_y, pg, dim3(256), 0, c->stream, y_dev, c->Yreal,
+                       (double2 *)c->Ycx, c->n, c->nblk, N);
+    // imaginary parts (Nyquist leakage), through the same buffer
+    if (c->nf_all > 0) {
+      if ((st = enqueue_all_factors(c, c->fl_imag, c->Yreal))) return st;
+      hipLaunchKernelGGL(rd::k_pack_y_imag, pg, dim3(256), 0, c->stream, c->Yreal,
+                         (double2 *)c->Ycx, c->n, c->nblk, c->ndim, N);
+    }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(Y_out, c->Ycx, ny * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   }

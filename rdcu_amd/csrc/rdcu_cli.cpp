@@ -15,7 +15,10 @@
 //   * The output-time loop checks its bound before reading t_eval
 //     (rdcu.cu:268 reads one past the end).
 //   * Integrator failures are reported and give exit status 1.
-//   * FILEBASEY.dat imaginary parts are 0.
+//   * FILEBASEY.dat imaginary parts hold the Nyquist leakage of the first
+//     and mixed derivative blocks, as the reference's complex transform
+//     does; block 0 (uninitialised memory in the reference) and the pure
+//     second derivatives (roundoff) hold 0.
 //   * FILEBASEfs.dat is replaced atomically (temporary file, then rename).
 
 #include <getopt.h>

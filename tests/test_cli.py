@@ -73,7 +73,10 @@ def test_output_files_layout(tmp_path):
     assert np.fromfile(tmp_path / "of.dat").size == nf * n * N
     Y = np.fromfile(tmp_path / "oY.dat").view(np.complex128).reshape(nf, 3, n, N)
     s = np.fromfile(tmp_path / "ostates.dat").reshape(nf, n, N)
-    assert np.array_equal(Y[:, 0].real, s) and np.all(Y.imag == 0)
+    assert np.array_equal(Y[:, 0].real, s) and np.all(Y[:, 0].imag == 0)
+    # 1-D: block 1 (d/dx) carries the Nyquist leakage in its imaginary part,
+    # block 2 (d2/dx2) none
+    assert np.any(Y[:, 1].imag != 0) and np.all(Y[:, 2].imag == 0)
     steps = np.fromfile(tmp_path / "osteps.dat")
     out = (tmp_path / "o.out").read_text()
     acc = int(out.split("steps: ")[1].split()[0])

@@ -53,7 +53,7 @@ def test_c2_full_size_bitwise(hlib, olib):
         fg, Yg = g.rhs(y0, factors=True)
         fo, Yo = o.rhs(y0, factors=True)
         assert np.array_equal(fg, fo)
-        assert np.array_equal(Yg.real, Yo.real)
+        assert np.array_equal(Yg.real, Yo.real) and np.array_equal(Yg.imag, Yo.imag)
         tg = to = 0.0
         hg = ho = 0.01
         for i in range(6):
