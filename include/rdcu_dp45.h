@@ -65,6 +65,31 @@ typedef struct rdcu_ctx rdcu_ctx;
 int rdcu_create(const rdcu_problem *prob, double atol, double rtol, int fixedstep,
                 const double *y0, int device, rdcu_ctx **out);
 
+/* A right-hand side given as code: the dydt callback of dp45_init (dp45.h:8,
+ * dp45.cu:171-177), called as dydt(t + c_s*h, ytemp, k_s) at every stage
+ * (dp45.cu:90-106) and dydt(t, y, k1) first (dp45.cu:157).  fn must make
+ * f_dev[0 .. n*N) = dydt(t, y_dev) with work ordered after everything already
+ * on `stream` (the integrator's hipStream_t) and complete, or ordered before
+ * anything enqueued on `stream` later, when it returns: the simplest correct
+ * callback synchronises `stream`, computes, and synchronises its own work.
+ * y_dev and f_dev are device memory (host memory in the CPU oracle, where
+ * stream is NULL).  A nonzero return aborts the call with RDCU_ECALLBACK. */
+typedef int (*rdcu_dydt_fn)(double t, const double *y_dev, double *f_dev, void *stream,
+                            void *user);
+
+/* Creates an integrator for a state of n*N values (n fields of N values; the
+ * error norm sums squares per field by the pairwise tree, then over the
+ * fields) whose right-hand side is fn.  Replaces dp45_init with an arbitrary
+ * dydt (dp45.h:8).  The same step control, run loop and dense output as a
+ * dictionary problem, on the device: rdcu_step, rdcu_run, rdcu_advance,
+ * rdcu_eval, rdcu_get_state, rdcu_set_state, rdcu_get_stats and rdcu_destroy
+ * take the handle; calls that need the dictionary (rdcu_rhs, rdcu_frame with
+ * f or Y, rdcu_time_rhs, coefficients and gradients) return
+ * RDCU_EUNSUPPORTED.  Attempts run one at a time, each with one host
+ * synchronisation (the callback sees t and h). */
+int rdcu_create_fn(int n, long long N, double atol, double rtol, int fixedstep, const double *y0,
+                   rdcu_dydt_fn fn, void *user, int device, rdcu_ctx **out);
+
 /* Frees everything.  Replaces dp45_destroy (dp45.h:10, dp45.cu:227-254). */
 int rdcu_destroy(rdcu_ctx *ctx);
 
@@ -125,6 +150,13 @@ int rdcu_set_state(rdcu_ctx *ctx, const double *y);
  * the first and mixed derivative blocks; 0 in block 0 (uninitialised memory
  * in the reference) and in the pure second derivatives (roundoff there). */
 int rdcu_rhs(rdcu_ctx *ctx, const double *y, double *f_out, double *Y_out);
+
+/* The dictionary RHS on device memory: f_dev[n*N] = dydt(y_dev[n*N]), the
+ * values of rdcu_rhs, complete when the call returns (the work runs on the
+ * handle's own stream).  For a callback of rdcu_create_fn that evaluates a
+ * dictionary, and for callers whose state already lives on the device.  The
+ * CPU oracle takes host memory. */
+int rdcu_rhs_device(rdcu_ctx *ctx, const double *y_dev, double *f_dev);
 
 /* rdcu_rhs that allocates nothing: the same values, bit for bit, with every
  * intermediate device buffer of the call (input staging, x-forward spectra,

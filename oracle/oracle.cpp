@@ -689,6 +689,9 @@ struct rdcu_ctx {
   double last_norm = 0;
   long long attempts = 0, accepted = 0;
   bool member = false;  // owned by a batch, stepped only through it
+  // right-hand side given as a callback (rdcu_create_fn): no dictionary
+  rdcu_dydt_fn fn = nullptr;
+  void *fn_user = nullptr;
 };
 
 // A batch is its instances stepped one after another (the sweep of
@@ -702,12 +705,30 @@ struct rdcu_batch {
 
 namespace {
 
+// Stage times of dp45.cu:90-106: t + h*c_s, and t + h for the last stage.
+const double STAGE_C[6] = {1.0 / 5, 3.0 / 10, 4.0 / 5, 8.0 / 9, 1.0, 1.0};
+
+// f = dydt(t, y): the dictionary, or the handle's callback.
+int rhs_at(rdcu_ctx *c, double t, const double *y, double *f) {
+  if (!c->fn) {
+    c->E.rhs(y, f);
+    return RDCU_OK;
+  }
+  if (c->fn(t, y, f, nullptr, c->fn_user))
+    return fail(RDCU_ECALLBACK, "the right-hand side callback returned an error");
+  return RDCU_OK;
+}
+
+int no_dict() {
+  return fail(RDCU_EUNSUPPORTED, "this call needs a dictionary problem (the handle's RHS is a callback)");
+}
+
 // one DP45 attempt (dp45.cu:87-147)
 int attempt(rdcu_ctx *c, double *t, double *h, double hmax, int *acc) {
   const Problem &P = c->P;
   const long nN = (long)P.n * P.N;
   if (!c->k1_valid) {
-    c->E.rhs(c->y.data(), c->k[0].data());
+    if (int st = rhs_at(c, *t, c->y.data(), c->k[0].data())) return st;
     c->k1_valid = true;
   }
   const double hh = *h;
@@ -723,7 +744,8 @@ int attempt(rdcu_ctx *c, double *t, double *h, double hmax, int *acc) {
         if (a[j] != 0.0) sacc = std::fma(a[j], c->k[j][i], sacc);
       yt[i] = std::fma(hh, sacc, y[i]);
     }
-    c->E.rhs(yt, c->k[s - 1].data());
+    const double ts = s < 7 ? *t + hh * STAGE_C[s - 2] : *t + hh;
+    if (int st = rhs_at(c, ts, yt, c->k[s - 1].data())) return st;
   }
   ++c->attempts;
   if (c->fixed) {  // dp45.cu:109-115
@@ -1014,6 +1036,37 @@ int rdcu_create(const rdcu_problem *prob, double atol, double rtol, int fixedste
   return RDCU_OK;
 }
 
+int rdcu_create_fn(int n, long long N, double atol, double rtol, int fixedstep, const double *y0,
+                   rdcu_dydt_fn fn, void *user, int device, rdcu_ctx **out) {
+  (void)device;
+  if (!out) return fail(RDCU_EINVAL, "null output handle");
+  *out = nullptr;
+  if (!fn) return fail(RDCU_EINVAL, "null right-hand side callback");
+  if (n < 1 || N < 1) return fail(RDCU_EINVAL, "n and N must be >= 1");
+  std::unique_ptr<rdcu_ctx> c(new rdcu_ctx());
+  c->fn = fn;
+  c->fn_user = user;
+  Problem &P = c->P;
+  P.n = n;
+  P.ndim = 1;
+  P.N = (long)N;
+  P.Ns[0] = (int)std::min<long long>(N, 1 << 30);
+  P.Nx = (int)P.Ns[0];
+  P.rows = 1;
+  P.nblk = 1;
+  c->atol = atol;
+  c->rtol = rtol;
+  c->fixed = fixedstep;
+  const size_t nN = (size_t)n * N;
+  c->y.assign(nN, 0.0);
+  if (y0) std::memcpy(c->y.data(), y0, nN * sizeof(double));
+  c->yt.assign(nN, 0.0);
+  c->yl.assign(nN, 0.0);
+  for (auto &kk : c->k) kk.assign(nN, 0.0);
+  *out = c.release();
+  return RDCU_OK;
+}
+
 int rdcu_destroy(rdcu_ctx *c) {
   delete c;
   return RDCU_OK;
@@ -1030,7 +1083,7 @@ int rdcu_run(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double
              rdcu_step_fn fn, void *user) {
   if (!c || !t || !h) return fail(RDCU_EINVAL, "null argument");
   if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
-  c->E.rhs(c->y.data(), c->k[0].data());  // dp45.cu:157
+  if (int st = rhs_at(c, *t, c->y.data(), c->k[0].data())) return st;  // dp45.cu:157
   c->k1_valid = true;
   while (*t < t1 && *h > hmin) {  // dp45.cu:159
     int acc = 0;
@@ -1046,7 +1099,7 @@ int rdcu_run(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double
 static int run_attempts(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double t1,
                         long long max_attempts, long long *done) {
   long long k = 0;
-  c->E.rhs(c->y.data(), c->k[0].data());
+  if (int st = rhs_at(c, *t, c->y.data(), c->k[0].data())) return st;
   c->k1_valid = true;
   while (k < max_attempts && *t < t1 && *h > hmin) {
     int acc = 0;
@@ -1121,6 +1174,7 @@ int rdcu_batch_run(rdcu_batch *b, double *t, double *h, double hmin, double hmax
 
 int rdcu_set_coef(rdcu_ctx *c, const double *coef) {
   if (!c || (!coef && !c->P.terms.empty())) return fail(RDCU_EINVAL, "null argument");
+  if (c->fn) return no_dict();
   for (size_t t = 0; t < c->P.terms.size(); ++t) c->P.terms[t].coef = coef[t];
   c->k1_valid = false;
   return RDCU_OK;
@@ -1129,6 +1183,7 @@ int rdcu_set_coef(rdcu_ctx *c, const double *coef) {
 int rdcu_schedule(rdcu_ctx *c, double t0, double h0, double hmin, double hmax, double t1,
                   long long max_steps, double *hs, long long *nsteps) {
   if (!c || !hs || !nsteps) return fail(RDCU_EINVAL, "null argument");
+  if (c->fn) return no_dict();
   if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
   double t = t0, h = h0;
   long long k = 0;
@@ -1153,6 +1208,7 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
                    double *grad_coef, double *grad_y0) {
   using adj::Vec;
   if (!c || !y0 || !hs || !t_out || !target || !loss) return fail(RDCU_EINVAL, "null argument");
+  if (c->fn) return no_dict();
   if (ns < 1 || nout < 1) return fail(RDCU_EINVAL, "need at least one step and one output");
   const long nN = (long)c->P.n * c->P.N;
   std::vector<long long> ostep;
@@ -1255,6 +1311,7 @@ int rdcu_loss_grad_batch(rdcu_grad_job *jobs, int count, int concurrency) {
 int rdcu_time_rhs(rdcu_ctx *c, int reps, double *ms, int max, int *count, char *names,
                   int names_len) {
   if (!c || !ms || max < 1 || reps < 1) return fail(RDCU_EINVAL, "bad argument");
+  if (c->fn) return no_dict();
   std::vector<double> f(c->y.size());
   double t0 = omp_get_wtime();
   for (int r = 0; r < reps; ++r) c->E.rhs(c->y.data(), f.data());
@@ -1314,6 +1371,7 @@ int rdcu_set_state(rdcu_ctx *c, const double *y) {
 
 int rdcu_rhs(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
   if (!c || !y) return fail(RDCU_EINVAL, "null argument");
+  if (c->fn) return no_dict();
   const Problem &P = c->P;
   const long N = P.N;
   const std::vector<Factor> &fl = Y_out ? c->E.all : c->E.used;
@@ -1359,6 +1417,12 @@ int rdcu_rhs(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
     }
   }
   return RDCU_OK;
+}
+
+// The CPU build has no device memory: the pointers are host memory.
+int rdcu_rhs_device(rdcu_ctx *c, const double *y_dev, double *f_dev) {
+  if (!c || !y_dev || !f_dev) return fail(RDCU_EINVAL, "null argument");
+  return rdcu_rhs(c, y_dev, f_dev, nullptr);
 }
 
 // The CPU build has no device scratch: the same values as rdcu_rhs.

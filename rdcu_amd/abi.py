@@ -34,7 +34,7 @@ SYMBOLS = (
     "rdcu_batch_create", "rdcu_batch_destroy", "rdcu_batch_step", "rdcu_batch_run",
     "rdcu_batch_member", "rdcu_set_coef", "rdcu_schedule", "rdcu_loss_grad",
     "rdcu_loss_grad_batch", "rdcu_frame", "rdcu_host_alloc", "rdcu_host_free",
-    "rdcu_rhs_noalloc",
+    "rdcu_rhs_noalloc", "rdcu_create_fn", "rdcu_rhs_device",
 )
 
 
@@ -64,6 +64,9 @@ class _GradJob(ctypes.Structure):
         ("target", _DP), ("loss", _DP), ("grad_coef", _DP), ("grad_y0", _DP),
         ("status", ctypes.c_int),
     ]
+# rdcu_dydt_fn: (t, y_dev, f_dev, stream, user) -> status
+DYDT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                           ctypes.c_void_p, ctypes.c_void_p)
 STEP_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                            ctypes.c_int, ctypes.c_int, _DP, ctypes.c_void_p)
 
@@ -95,6 +98,10 @@ def load(path: str) -> ctypes.CDLL:
     lib.rdcu_set_state.argtypes = [vp, _DP]
     lib.rdcu_rhs.argtypes = [vp, _DP, _DP, _DP]
     lib.rdcu_rhs_noalloc.argtypes = [vp, _DP, _DP, _DP]
+    lib.rdcu_rhs_device.argtypes = [vp, vp, vp]
+    lib.rdcu_create_fn.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int, _DP, DYDT_FN, vp, ctypes.c_int,
+                                   ctypes.POINTER(vp)]
     lib.rdcu_frame.argtypes = [vp, ctypes.c_double, _DP, _DP, _DP]
     lib.rdcu_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
     lib.rdcu_host_free.argtypes = [vp]
@@ -385,6 +392,47 @@ class _Member(Integrator):
 
     def close(self):
         self.h = None
+
+
+class CallbackIntegrator(Integrator):
+    """DP45 on a right-hand side given as a Python callable (rdcu_create_fn):
+    the dydt callback of dp45_init (dp45.h:8).  ``fn(t, y_ptr, f_ptr,
+    stream)`` receives raw pointers -- device memory on the HIP library (the
+    integrator's stream is ``stream``; the callable must leave f complete or
+    ordered before later work on it), host memory on the CPU oracle -- and
+    returns 0 (nonzero aborts).  The state is n fields of N values.  step,
+    run, advance, eval, state, set_state and stats work as for a dictionary
+    problem."""
+
+    def __init__(self, lib: ctypes.CDLL, n: int, N: int, y0: np.ndarray, fn: Callable,
+                 atol=1e-8, rtol=1e-8, fixed=False, device=0):
+        self.lib = lib
+        self.prob = None
+        self.nN = n * N
+        y0 = np.ascontiguousarray(y0, dtype=np.float64).reshape(-1)
+        if y0.size != self.nN:
+            raise ValueError(f"y0 has {y0.size} values, expected {self.nN}")
+        self._err = []
+
+        def cb(t, y, f, stream, user):
+            try:
+                return int(fn(t, y, f, stream) or 0)
+            except Exception as e:  # surfaced after the call
+                self._err.append(e)
+                return 1
+
+        self._fn = DYDT_FN(cb)  # kept alive with the handle
+        h = ctypes.c_void_p()
+        self._check(lib.rdcu_create_fn(n, N, atol, rtol, int(bool(fixed)), _ptr(y0), self._fn,
+                                       None, device, ctypes.byref(h)))
+        self.h = h
+
+    def _check(self, code):
+        if getattr(self, "_err", None):
+            e = self._err[0]
+            self._err.clear()
+            raise e
+        super()._check(code)
 
 
 class Batch:

@@ -301,6 +301,7 @@ extern "C" {
 
 int rdcu_set_coef(rdcu_ctx *c, const double *coef) {
   if (!c || (!coef && !c->h_tcoef.empty())) return fail(RDCU_EINVAL, "null argument");
+  if (c->fn) return no_dict();
   HIPCHECK(hipSetDevice(c->dev));
   const int nt = (int)c->h_tcoef.size();
   for (int i = 0; i < nt; ++i) c->h_tcoef[i] = coef[i];
@@ -326,6 +327,7 @@ int rdcu_set_coef(rdcu_ctx *c, const double *coef) {
 int rdcu_schedule(rdcu_ctx *c, double t0, double h0, double hmin, double hmax, double t1,
                   long long max_steps, double *hs, long long *nsteps) {
   if (!c || !hs || !nsteps) return fail(RDCU_EINVAL, "null argument");
+  if (c->fn) return no_dict();
   if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
   HIPCHECK(hipSetDevice(c->dev));
   int st;
@@ -359,6 +361,7 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
                    int nout, const double *t_out, const double *target, double *loss,
                    double *grad_coef, double *grad_y0) {
   if (!c || !y0 || !hs || !t_out || !target || !loss) return fail(RDCU_EINVAL, "null argument");
+  if (c->fn) return no_dict();
   if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
   if (ns < 1 || nout < 1) return fail(RDCU_EINVAL, "need at least one step and one output");
   if (!c->reg) return fail(RDCU_EUNSUPPORTED, "gradients need transform lengths 2^k or 3*2^k");

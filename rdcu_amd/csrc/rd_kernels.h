@@ -687,6 +687,67 @@ __global__ void k_control(DevState *st, const double *__restrict__ part, long cn
   control_body(st, part, cnt, nN);
 }
 
+// ----------------------------------------------------------- callback RHS
+// A right-hand side given as a host callback (dp45_init's dydt, dp45.h:8)
+// instead of a dictionary: the stage sums, the error norm and the step
+// control run here, the RHS wherever the callback puts it.  The state is n
+// lines (fields) of N values; the norm sums squares per line by the pairwise
+// tree, then over the lines -- the tree of a 1-D problem of n fields.
+
+// Values of one line per block of the error pass: a power of two, so every
+// block sums an aligned subtree of the line's pairwise tree.
+constexpr int kFnChunk = kRedCap;
+
+// Stage input ytemp = y + h * (sum_j a_j k_j), zero coefficients skipped:
+// the operations of k_rowf_reg's stage sum (dp45.cu:34-69).
+__global__ void k_fn_stage(const DevState *st, Bufs b, StageCoef sc, long nN) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nN) return;
+  const double h = st->h;
+  double acc = sc.a[0] * b.k[st->k1s][i];
+  for (int j = 1; j < sc.ns; ++j)
+    if (sc.a[j] != 0.0) acc = __builtin_fma(sc.a[j], b.k[j][i], acc);
+  b.y[st->iyt][i] = __builtin_fma(h, acc, b.y[st->iy][i]);
+}
+
+// Scaled error squares of chunk (line, chunk) summed by its subtree
+// (dp45.cu:79-84): the operations of the x-inverse pass's error tree.
+__global__ void k_fn_err(const DevState *st, Bufs b, ErrCoef ec, long N, int nchunk,
+                         double *out) {
+  __shared__ double v[kFnChunk];
+  const int line = blockIdx.x / nchunk, ch = blockIdx.x - line * nchunk;
+  const long off = (long)ch * kFnChunk;
+  const int c = (int)((N - off) < kFnChunk ? (N - off) : kFnChunk);
+  const long base = (long)line * N + off;
+  const double h = st->h, atol = st->atol, rtol = st->rtol;
+  const double *y = b.y[st->iy], *yn = b.y[st->iyt], *k1 = b.k[st->k1s], *k7 = b.k[st->k7s];
+  for (int x = threadIdx.x; x < c; x += blockDim.x) {
+    const long i = base + x;
+    double e = ec.e[0] * k1[i];
+    e = __builtin_fma(ec.e[2], b.k[2][i], e);
+    e = __builtin_fma(ec.e[3], b.k[3][i], e);
+    e = __builtin_fma(ec.e[4], b.k[4][i], e);
+    e = __builtin_fma(ec.e[5], b.k[5][i], e);
+    e = __builtin_fma(ec.e[6], k7[i], e);
+    const double w = __builtin_fma(rtol, fmax(fabs(y[i]), fabs(yn[i])), atol);
+    const double q = (h * e) / w;
+    v[x] = q * q;
+  }
+  __syncthreads();
+  lds_pairsum(v, c, threadIdx.x, blockDim.x);
+  if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+}
+
+// Per line, the pairwise tree over its chunk sums (block = line).
+__global__ void k_fn_lines(const double *__restrict__ in, int nchunk, double *out) {
+  __shared__ double v[kRedCap];
+  const double *p = in + (long)blockIdx.x * nchunk;
+  for (int i = threadIdx.x; i < nchunk; i += blockDim.x) v[i] = p[i];
+  __syncthreads();
+  lds_pairsum(v, nchunk, threadIdx.x, blockDim.x);
+  if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+}
+
 // ---------------------------------------------------------------- batches
 // One instance of a batch, as the batched kernels see it (a device table of
 // these; each instance owns its buffers, state, factor list and dictionary).

@@ -212,6 +212,10 @@ struct rdcu_ctx {
   std::vector<int> h_ublk, h_ufld, h_prog;  // referenced factors; dictionary program
   // adjoint: companion problem applying the derivative blocks to slot weights
   std::unique_ptr<rdcu_ctx> adj_pc;
+  // right-hand side given as a callback (rdcu_create_fn): no dictionary
+  rdcu_dydt_fn fn = nullptr;
+  void *fn_user = nullptr;
+  int fn_nchunk = 1;
 
   ~rdcu_ctx() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -1129,6 +1133,14 @@ int launch_attempts(rdcu_ctx *c, long long K) {
   return RDCU_OK;
 }
 
+int no_dict() {
+  return fail(RDCU_EUNSUPPORTED, "this call needs a dictionary problem (the handle's RHS is a callback)");
+}
+
+int fn_step(rdcu_ctx *c, double *t, double *h, double hmax, int *accepted);
+int fn_run(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double t1, double stop,
+           rdcu_step_fn cb, void *user, long long max_attempts, long long *done);
+
 int read_state(rdcu_ctx *c) {
   HIPCHECK(hipMemcpyAsync(c->sh, c->st, sizeof(rd::DevState), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1548,6 +1560,7 @@ int rdcu_step(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, int *
   if (!c || !t || !h || !accepted) return fail(RDCU_EINVAL, "null argument");
   if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
   HIPCHECK(hipSetDevice(c->dev));
+  if (c->fn) return fn_step(c, t, h, hmax, accepted);
   int st;
   if ((st = begin(c, INFINITY, 1, *t, *h, -INFINITY, hmax, INFINITY))) return st;
   if ((st = ensure_k1(c))) return st;
@@ -1566,6 +1579,7 @@ int rdcu_run(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, double
   if (!c || !t || !h) return fail(RDCU_EINVAL, "null argument");
   if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
   HIPCHECK(hipSetDevice(c->dev));
+  if (c->fn) return fn_run(c, t, h, hmin, hmax, t1, stop, fn, user, -1, nullptr);
   int st;
   if ((st = begin(c, stop, 1, *t, *h, hmin, hmax, t1))) return st;
   c->k1_valid = false;  // dp45_run recomputes k1 = f(y) (dp45.cu:157)
@@ -1606,6 +1620,7 @@ int rdcu_advance(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, do
   if (!c || !t || !h) return fail(RDCU_EINVAL, "null argument");
   if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
   HIPCHECK(hipSetDevice(c->dev));
+  if (c->fn) return fn_run(c, t, h, hmin, hmax, t1, INFINITY, nullptr, nullptr, max_attempts, done);
   int st;
   if ((st = begin(c, INFINITY, 1, *t, *h, hmin, hmax, t1))) return st;
   c->k1_valid = false;
@@ -1633,6 +1648,7 @@ int rdcu_advance(rdcu_ctx *c, double *t, double *h, double hmin, double hmax, do
 int rdcu_time_rhs(rdcu_ctx *c, int reps, double *ms, int max, int *count, char *names,
                   int names_len) {
   if (!c || !ms || max < 1 || reps < 1) return fail(RDCU_EINVAL, "bad argument");
+  if (c->fn) return no_dict();
   HIPCHECK(hipSetDevice(c->dev));
   auto ls = c->reg ? rhs_launches_reg(c, rd::SRC_Y, nullptr, rd::DST_PTR, 0, c->rhs_out, 0)
                    : rhs_launches(c, 0, rd::SRC_Y, nullptr, rd::DST_PTR, 0, c->rhs_out, 0, 0);
@@ -1735,6 +1751,7 @@ int enqueue_rhs_out(rdcu_ctx *c, const double *y_dev, double *f_out, double *Y_o
 
 int rdcu_rhs(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
   if (!c || !y) return fail(RDCU_EINVAL, "null argument");
+  if (c->fn) return no_dict();
   if (Y_out && !c->lds_y) return fail(RDCU_EUNSUPPORTED, "factor output not available for lines this long");
   HIPCHECK(hipSetDevice(c->dev));
   const long nN = (long)c->n * c->N;
@@ -1747,6 +1764,7 @@ int rdcu_rhs(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
 
 int rdcu_rhs_noalloc(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
   if (!c || !y) return fail(RDCU_EINVAL, "null argument");
+  if (c->fn) return no_dict();
   if (Y_out && !c->lds_y) return fail(RDCU_EUNSUPPORTED, "factor output not available for lines this long");
   HIPCHECK(hipSetDevice(c->dev));
   char *base = nullptr;
@@ -1801,6 +1819,7 @@ int rdcu_rhs_noalloc(rdcu_ctx *c, const double *y, double *f_out, double *Y_out)
 
 int rdcu_frame(rdcu_ctx *c, double t_eval, double *y_out, double *f_out, double *Y_out) {
   if (!c || !y_out) return fail(RDCU_EINVAL, "null argument");
+  if (c->fn && (f_out || Y_out)) return no_dict();
   if (!c->sh->eval_valid) return fail(RDCU_ESTATE, "no accepted step to interpolate");
   if (Y_out && !c->lds_y) return fail(RDCU_EUNSUPPORTED, "factor output not available for lines this long");
   HIPCHECK(hipSetDevice(c->dev));
@@ -1849,3 +1868,4 @@ int rdcu_get_stats(rdcu_ctx *c, long long out[4]) {
 
 #include "rd_batch.hip"
 #include "rd_adjoint.hip"
+#include "rd_callback.hip"
