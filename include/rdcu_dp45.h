@@ -248,6 +248,16 @@ int rdcu_loss_grad(rdcu_ctx *ctx, const double *y0, double t0, long long nsteps,
                    const double *hs, int nout, const double *t_out, const double *target,
                    double *loss, double *grad_coef, double *grad_y0);
 
+/* rdcu_loss_grad with y0_dev[n*N], target_dev[nout*n*N] and grad_y0_dev[n*N]
+ * in device memory (target read in place; the small arrays hs, t_out, loss
+ * and grad_coef stay in host memory): the same values, for callers whose
+ * trajectories live on the device (torch tensors).  Complete on return.  The
+ * CPU oracle takes host memory. */
+int rdcu_loss_grad_device(rdcu_ctx *ctx, const double *y0_dev, double t0, long long nsteps,
+                          const double *hs, int nout, const double *t_out,
+                          const double *target_dev, double *loss, double *grad_coef,
+                          double *grad_y0_dev);
+
 /* One instance of a gradient batch: the arguments of rdcu_loss_grad, and its
  * status (an RDCU_* code) on return. */
 typedef struct rdcu_grad_job {

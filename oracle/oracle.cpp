@@ -1420,6 +1420,14 @@ int rdcu_rhs(rdcu_ctx *c, const double *y, double *f_out, double *Y_out) {
 }
 
 // The CPU build has no device memory: the pointers are host memory.
+int rdcu_loss_grad_device(rdcu_ctx *c, const double *y0_dev, double t0, long long ns,
+                          const double *hs, int nout, const double *t_out,
+                          const double *target_dev, double *loss, double *grad_coef,
+                          double *grad_y0_dev) {
+  return rdcu_loss_grad(c, y0_dev, t0, ns, hs, nout, t_out, target_dev, loss, grad_coef,
+                        grad_y0_dev);
+}
+
 int rdcu_rhs_device(rdcu_ctx *c, const double *y_dev, double *f_dev) {
   if (!c || !y_dev || !f_dev) return fail(RDCU_EINVAL, "null argument");
   return rdcu_rhs(c, y_dev, f_dev, nullptr);

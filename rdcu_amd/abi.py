@@ -34,7 +34,7 @@ SYMBOLS = (
     "rdcu_batch_create", "rdcu_batch_destroy", "rdcu_batch_step", "rdcu_batch_run",
     "rdcu_batch_member", "rdcu_set_coef", "rdcu_schedule", "rdcu_loss_grad",
     "rdcu_loss_grad_batch", "rdcu_frame", "rdcu_host_alloc", "rdcu_host_free",
-    "rdcu_rhs_noalloc", "rdcu_create_fn", "rdcu_rhs_device",
+    "rdcu_rhs_noalloc", "rdcu_create_fn", "rdcu_rhs_device", "rdcu_loss_grad_device",
 )
 
 
@@ -121,6 +121,8 @@ def load(path: str) -> ctypes.CDLL:
                                   ctypes.POINTER(ctypes.c_longlong)]
     lib.rdcu_loss_grad.argtypes = [vp, _DP, ctypes.c_double, ctypes.c_longlong, _DP, ctypes.c_int,
                                    _DP, _DP, _DP, _DP, _DP]
+    lib.rdcu_loss_grad_device.argtypes = [vp, vp, ctypes.c_double, ctypes.c_longlong, _DP,
+                                          ctypes.c_int, _DP, vp, _DP, _DP, vp]
     lib.rdcu_loss_grad_batch.argtypes = [ctypes.POINTER(_GradJob), ctypes.c_int, ctypes.c_int]
     lib.rdcu_last_error.restype = ctypes.c_char_p
     lib.rdcu_backend.restype = ctypes.c_char_p

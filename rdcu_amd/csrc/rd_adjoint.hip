@@ -357,9 +357,16 @@ int rdcu_schedule(rdcu_ctx *c, double t0, double h0, double hmin, double hmax, d
   return RDCU_OK;
 }
 
-int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const double *hs,
-                   int nout, const double *t_out, const double *target, double *loss,
-                   double *grad_coef, double *grad_y0) {
+}  // extern "C"
+
+namespace {
+
+// rdcu_loss_grad with y0, target and grad_y0 in host memory (dev = false) or
+// device memory (dev = true: target is read in place, y0 and the gradient
+// are copied device to device).
+int loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const double *hs, int nout,
+              const double *t_out, const double *target, double *loss, double *grad_coef,
+              double *grad_y0, bool dev) {
   if (!c || !y0 || !hs || !t_out || !target || !loss) return fail(RDCU_EINVAL, "null argument");
   if (c->fn) return no_dict();
   if (c->member) return fail(RDCU_ESTATE, "a batch member steps only through its batch");
@@ -377,9 +384,9 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
   const unsigned nbl = blocks_of(nN);
   DevScratch ds(c->stream);
   double *d_hs, *d_tg, *d_part, *ych = nullptr, *gbar = nullptr;
-  if ((st = ds.get(&d_hs, ns)) || (st = ds.get(&d_tg, (size_t)nout * nN)) ||
-      (st = ds.get(&d_part, (size_t)nout * nbl)))
-    return st;
+  if ((st = ds.get(&d_hs, ns)) || (st = ds.get(&d_part, (size_t)nout * nbl))) return st;
+  if (dev) d_tg = const_cast<double *>(target);
+  else if ((st = ds.get(&d_tg, (size_t)nout * nN))) return st;
   if (grad && ((st = ds.get(&ych, (size_t)ns * nN)) || (st = ds.get(&gbar, (size_t)nout * nN))))
     return st;
   // The stage values of every step are kept from the forward pass when they
@@ -395,9 +402,13 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
       return st;
   }
   HIPCHECK(hipMemcpyAsync(d_hs, hs, ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHECK(hipMemcpyAsync(d_tg, target, (size_t)nout * nN * sizeof(double), hipMemcpyHostToDevice,
-                          c->stream));
-  if ((st = rdcu_set_state(c, y0))) return st;
+  if (!dev)
+    HIPCHECK(hipMemcpyAsync(d_tg, target, (size_t)nout * nN * sizeof(double),
+                            hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->bufs.y[c->sh->iy], y0, nN * sizeof(double),
+                          dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+  c->k1_valid = false;
+  c->sh->eval_valid = 0;
 
   // ---- forward on the frozen schedule
   hipLaunchKernelGGL(rd::k_set_sched, dim3(1), dim3(1), 0, c->stream, c->st, 1, d_hs, ns, 1);
@@ -634,7 +645,8 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
   HIPCHECK(hipMemcpyAsync(hp.data(), P, hp.size() * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
   if (grad_y0)
-    HIPCHECK(hipMemcpyAsync(grad_y0, lam, nN * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(grad_y0, lam, nN * sizeof(double),
+                            dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   if (grad_coef)
     for (int t = 0; t < nt; ++t) {
@@ -643,6 +655,24 @@ int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const
       grad_coef[t] = sacc;
     }
   return RDCU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rdcu_loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const double *hs,
+                   int nout, const double *t_out, const double *target, double *loss,
+                   double *grad_coef, double *grad_y0) {
+  return loss_grad(c, y0, t0, ns, hs, nout, t_out, target, loss, grad_coef, grad_y0, false);
+}
+
+int rdcu_loss_grad_device(rdcu_ctx *c, const double *y0_dev, double t0, long long ns,
+                          const double *hs, int nout, const double *t_out,
+                          const double *target_dev, double *loss, double *grad_coef,
+                          double *grad_y0_dev) {
+  return loss_grad(c, y0_dev, t0, ns, hs, nout, t_out, target_dev, loss, grad_coef, grad_y0_dev,
+                   true);
 }
 
 int rdcu_loss_grad_batch(rdcu_grad_job *jobs, int count, int concurrency) {
