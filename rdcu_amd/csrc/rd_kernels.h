@@ -325,6 +325,7 @@ struct ColGeom {
   int axis;    // grid axis of the lines (multiplier index)
   int nparts;  // blocks per tile in mode 1 (factor split, register path)
   int nblk;    // logical blocks (register path; the grid is rounded up to 8)
+  int bal;     // register path: tiles of even widths (1) or of tc from column 0 (0)
 };
 
 // Logical block of a launch whose grid is a multiple of the 8 XCDs: hardware

@@ -285,13 +285,15 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   }
   const bool idle = mode == 1 && m0 >= m1;
   const int L = LS > 0 ? LS : cg.L, T = L / E, tc = cg.tc;
-  // the Nxh columns of a group are split into ntc tiles whose widths differ
-  // by at most one (tc >= the widest), so that a launch can be sized to the
-  // CU count: at 512^2 the 257 columns of a field fill 64 tiles of 4 or 5
-  // instead of 64 tiles of 4 plus a 65th of one column, which put a second
-  // block on one CU of every XCD
-  const int c0 = (int)((long)ct * g.Nxh / cg.ntc);
-  const int nc = (int)((long)(ct + 1) * g.Nxh / cg.ntc) - c0;
+  // tiles of tc columns from column 0 (whole 16-byte-column groups per
+  // cache line; the last tile holds the remainder), or with bal the Nxh
+  // columns of a group split into ntc tiles whose widths differ by at most
+  // one (tc >= the widest), so that a launch can be sized to the CU count:
+  // at 512^2 the 257 columns of a field fill 64 tiles of 4 or 5 instead of
+  // 64 tiles of 4 plus a 65th of one column
+  const int c0 = cg.bal ? (int)((long)ct * g.Nxh / cg.ntc) : ct * tc;
+  const int nc = cg.bal ? (int)((long)(ct + 1) * g.Nxh / cg.ntc) - c0
+                        : ((g.Nxh - c0) < tc ? (g.Nxh - c0) : tc);
   const int tid = threadIdx.x;
   const int c = tid % tc, t = tid / tc;
   const bool act = c < nc;
@@ -326,62 +328,37 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     }
     blk0 = idle ? 0 : fl.fblk[m0];
   }
-  // lines of a compile-time length with 8 values per thread keep their
-  // twiddles and axis-0 wavenumbers in registers, loaded here with the
-  // column (no LDS copies, no LDS reads of them in the passes)
-  constexpr bool RT = LS > 0 && !TG && E == 8;
-  constexpr int NTW = RT ? static_tw_count<(LS > 0 ? LS : 8)>(E) : 1;
-  cx twr[NTW];
-  double k0r[RT ? E : 1];
   Prefetch<4, cx> pf_tw;
   Prefetch<4, double> pf_k;
-  if constexpr (RT) {
-    load_tw_s<E, (LS > 0 ? LS : 8)>(twr, p.tw, t);
-    if (mode == 1) {
-#pragma unroll
-      for (int q = 0; q < E; ++q) k0r[q] = g.kt[0][t + T * q];
-    }
-  } else if constexpr (!TG) {
+  if constexpr (!TG) {
     pf_tw.load(p.tw, L, tid, blockDim.x);
     if (mode == 1) pf_k.load(g.kt[0], L, tid, blockDim.x);
   }
   if (idle || (chk && !attempt_active(chk))) return;
-  if constexpr (!TG && !RT) {
+  if constexpr (!TG) {
     pf_tw.store(twl, p.tw, L, tid, blockDim.x);
     if (mode == 1) pf_k.store(k0l, g.kt[0], L, tid, blockDim.x);
   }
   cx *lb = buf + c;
-  auto fft_f = [&](cx(&x)[E]) {
-    if constexpr (RT) reg_fft_w<E, -1, (LS > 0 ? LS : 8)>(x, lb, tc, twr, t);
-    else fft_any<E, -1, LS>(x, lb, tc, tw, p, t, T);
-  };
-  auto fft_i = [&](cx(&x)[E]) {
-    if constexpr (RT) reg_fft_w<E, +1, (LS > 0 ? LS : 8)>(x, lb, tc, twr, t);
-    else fft_any<E, +1, LS>(x, lb, tc, tw, p, t, T);
-  };
-  auto k0_of = [&](int q) -> double {
-    if constexpr (RT) return k0r[q];
-    else return k0s[t + T * q];
-  };
   if (mode != 1) {
-    if (mode == 0) fft_f(v);
-    else fft_i(v);
+    if (mode == 0) fft_any<E, -1, LS>(v, lb, tc, tw, p, t, T);
+    else fft_any<E, +1, LS>(v, lb, tc, tw, p, t, T);
     if (act) {
 #pragma unroll
       for (int q = 0; q < E; ++q) st_g<NT>(src + (long)(t + T * q) * cg.es, v[q]);
     }
     return;
   }
-  fft_f(v);
+  fft_any<E, -1, LS>(v, lb, tc, tw, p, t, T);
   if (!keep) {  // one factor: the spectrum goes from registers to its multiply
     cx w[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       nk[0] = t + T * q;
-      kv[0] = k0_of(q);
+      kv[0] = k0s[t + T * q];
       w[q] = mult_k(g, blk0, nk, kv, v[q]);
     }
-    fft_i(w);
+    fft_any<E, +1, LS>(w, lb, tc, tw, p, t, T);
     if (act) {
       cx *dst = b.F + (long)m0 * g.Nh + lbase;
 #pragma unroll
@@ -400,10 +377,10 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       nk[0] = t + T * q;
-      kv[0] = k0_of(q);
+      kv[0] = k0s[t + T * q];
       w[q] = mult_k(g, blk, nk, kv, xs[(t + T * q) * tc + c]);
     }
-    fft_i(w);
+    fft_any<E, +1, LS>(w, lb, tc, tw, p, t, T);
     if (act) {
       cx *dst = b.F + (long)m * g.Nh + lbase;
 #pragma unroll

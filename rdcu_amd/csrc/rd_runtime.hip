@@ -197,7 +197,7 @@ struct rdcu_ctx {
   // strided axes (0 = not eligible); shapes picked by the autotuner
   int reg = 0, Ex = 0, E0 = 0, E1 = 0;
   int rf_rpb = 1, cr_tc = 1, ri_G = 1, ri_R = 1, cr_fpw = 1, max_fpf = 1;
-  int cr_ntc = 0;  // strided tiles per column group (0: ceil(Nxh / cr_tc))
+  int cr_ntc = 0;  // strided tiles of even widths per column group (0: tiles of cr_tc)
   size_t lds_rf = 0, lds_cr0 = 0, lds_cr1 = 0, lds_ri = 0;
   bool lds_rhs = true, lds_y = true;  // the LDS kernels fit (see choose_shapes)
   std::string tune_log;
@@ -537,12 +537,16 @@ rd::StageCoef stage_coef(int stage) {
   return sc;
 }
 
-// Column tiles of the strided passes: ntc tiles per group (0: as many
-// tiles of tc columns as the Nxh columns need), each block as wide as the
-// widest tile (the kernel splits the columns evenly over the tiles).
-void col_tiles(const rdcu_ctx *c, int &tc, int &ntc) {
-  if (ntc <= 0 || ntc > c->Nxh) ntc = (c->Nxh + tc - 1) / tc;
+// Column tiles of the strided passes: with ntc > 0, ntc tiles of even
+// widths per group, each block as wide as the widest (bal = 1); otherwise
+// tiles of tc columns from column 0 (bal = 0, ntc = as many as needed).
+int col_tiles(const rdcu_ctx *c, int &tc, int &ntc) {
+  if (ntc <= 0 || ntc > c->Nxh) {
+    ntc = (c->Nxh + tc - 1) / tc;
+    return 0;
+  }
   tc = (c->Nxh + ntc - 1) / ntc;
+  return 1;
 }
 
 // Strided passes of the register path for one RHS (after the x forward pass).
@@ -554,9 +558,10 @@ std::vector<Launch> col_launches_reg(rdcu_ctx *c, int check, int tc, int ntc) {
     return v;
   }
   const int nd = c->ndim;
-  col_tiles(c, tc, ntc);
+  const int bal = col_tiles(c, tc, ntc);
   if (nd == 3) {
     rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc, tc, 1};
+    cg.bal = bal;
     long ng = (long)c->n * c->Ns[0];
     v.push_back({"col_fwd", [=]() { colr(c, c->p1, c->E1, cg, ng, check, c->fl_used, 0); }});
   }
@@ -565,6 +570,7 @@ std::vector<Launch> col_launches_reg(rdcu_ctx *c, int check, int tc, int ntc) {
     cg.L = c->Ns[0];
     cg.ntc = ntc;
     cg.tc = tc;
+    cg.bal = bal;
     cg.gs1 = c->Nh;
     if (nd == 2) {
       cg.es = c->Nxp;
@@ -579,6 +585,7 @@ std::vector<Launch> col_launches_reg(rdcu_ctx *c, int check, int tc, int ntc) {
   }
   if (nd == 3 && c->nf_used > 0) {
     rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc, tc, 1};
+    cg.bal = bal;
     long ng = (long)c->nf_used * c->Ns[0];
     v.push_back({"col_inv", [=]() { colr(c, c->p1, c->E1, cg, ng, check, c->fl_used, 2); }});
   }
@@ -858,7 +865,7 @@ int fixed_shapes(rdcu_ctx *c, const char *spec) {
   int rpb = 0, tc = 0, fpw = 0, G = 0, R = 1, Ec = 0, ntc = 0;
   const int got = sscanf(spec, "%d,%d,%d,%d,%d,%d,%d", &rpb, &tc, &fpw, &G, &R, &Ec, &ntc);
   if (got < 4) return fail(RDCU_EINVAL, "RDCU_SHAPES must be rpb,tc,fpw,G[,R[,Ecol[,ntc]]]");
-  if (c->ndim >= 2 && tc >= 1) col_tiles(c, tc, ntc);
+  if (c->ndim >= 2 && tc >= 1 && col_tiles(c, tc, ntc) == 0) ntc = 0;
   if (Ec == 16) {  // 16 values per thread on the power-of-two strided axes
     if (c->E0 == 8 && c->Ns[0] >= 16) c->E0 = 16;
     if (c->ndim == 3 && c->E1 == 8 && c->Ns[1] >= 16) c->E1 = 16;
@@ -946,11 +953,12 @@ int autotune(rdcu_ctx *c) {
         // tiles of tc0 columns (the last one holding the remainder), and the
         // same columns in one tile fewer, spread evenly (blocks one column
         // wider), so that the block count can land on a multiple of the CUs
-        int ntcs[2] = {(c->Nxh + tc0 - 1) / tc0, c->Nxh / tc0};
+        const int full = (c->Nxh + tc0 - 1) / tc0;
         for (int k = 0; k < 2; ++k) {
-          int tc = tc0, ntc = ntcs[k];
-          if (ntc < 1 || (k == 1 && ntc == ntcs[0])) continue;
+          int tc = tc0, ntc = k == 0 ? 0 : c->Nxh / tc0;
+          if (k == 1 && (ntc < 1 || ntc == full)) continue;
           col_tiles(c, tc, ntc);
+          if (k == 0) ntc = 0;  // tiles of tc0 from column 0
           if (tc * Tmax > col_threads(c) || !col_fits(c, tc, 1)) continue;
           for (int fpw = c->max_fpf; fpw >= 1; fpw = fpw > 1 ? (fpw + 1) / 2 : 0) {
             if (!col_fits(c, tc, fpw)) continue;
