@@ -341,6 +341,88 @@ __device__ __forceinline__ void reg_fft_s(cx (&v)[E], cx *buf, int es, const cx 
   }
 }
 
+// The same compile-time transform with the twiddles in registers: every
+// twiddle a thread multiplies by (passes after the first, in pass order,
+// butterfly by butterfly, r = 1..R-1) is loaded from the table once, at the
+// start of the kernel, together with its first data loads, instead of being
+// read from LDS in every pass.  The values and the arithmetic are those of
+// reg_fft_s, so the results are identical; the inverse conjugates at use.
+template <int L, int IP = 0>
+constexpr int static_tw_count(int E) {
+  if constexpr (IP >= static_plan_np(L)) {
+    return 0;
+  } else {
+    constexpr int R = static_plan_rad(L, IP);
+    return (IP > 0 ? (E / R) * (R - 1) : 0) + static_tw_count<L, IP + 1>(E);
+  }
+}
+
+template <int E, int L, int IP = 0, int NS = 1, int OFF = 0>
+__device__ __forceinline__ void load_tw_s(cx *w, const cx *__restrict__ tw, int t) {
+  constexpr int NP = static_plan_np(L);
+  if constexpr (IP < NP) {
+    constexpr int R = static_plan_rad(L, IP);
+    constexpr int T = L / E, B = E / R, TWS = L / (NS * R);
+    if constexpr (IP > 0) {
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const int k = (t + T * b) % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[OFF + b * (R - 1) + r - 1] = tw[(k * r) * TWS];
+      }
+    }
+    load_tw_s<E, L, IP + 1, NS * R, OFF + (IP > 0 ? B * (R - 1) : 0)>(w, tw, t);
+  }
+}
+
+template <int R, int E, int DIR, int L, int NS, bool LAST>
+__device__ __forceinline__ void reg_pass_w(cx (&v)[E], cx *buf, int es, const cx *w, int t) {
+  constexpr int B = E / R, T = L / E;
+  static_assert(E % R == 0, "values per thread must be a multiple of every radix");
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = t + T * b;
+    const int k = NS == 1 ? 0 : j % NS;
+    cx u[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) u[r] = v[b + B * r];
+    if (NS > 1 && k != 0) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cx wv = w[b * (R - 1) + r - 1];
+        if constexpr (DIR > 0) wv.im = -wv.im;
+        u[r] = cmul(u[r], wv);
+      }
+    }
+    dft<R, DIR>(u);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + B * r] = u[r];
+    } else {
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[pad8(base + r * NS) * es] = u[r];
+    }
+  }
+}
+
+template <int E, int DIR, int L, int IP = 0, int NS = 1, int OFF = 0>
+__device__ __forceinline__ void reg_fft_w(cx (&v)[E], cx *buf, int es, const cx *w, int t) {
+  constexpr int NP = static_plan_np(L);
+  if constexpr (IP < NP) {
+    constexpr int R = static_plan_rad(L, IP);
+    constexpr int T = L / E;
+    if constexpr (IP > 0) {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = buf[pad8(t + T * q) * es];
+      __syncthreads();
+    }
+    reg_pass_w<R, E, DIR, L, NS, IP == NP - 1>(v, buf, es, w + OFF, t);
+    reg_fft_w<E, DIR, L, IP + 1, NS * R, OFF + (IP > 0 ? (E / R) * (R - 1) : 0)>(v, buf, es, w, t);
+  }
+}
+
 // reg_fft with the length fixed at compile time when LS > 0 (LS must then
 // equal p.L), the run-time plan otherwise.
 template <int E, int DIR, int LS>

@@ -482,7 +482,7 @@ __host__ __device__ inline long rowi_region(int L, int Nx, int n, int nf, int nw
 // One grid row r, by the nt threads tid = 0..nt-1 of a row group whose LDS
 // region starts at smem.  Every row group of a block runs the same barrier
 // sequence.
-template <int E, class DE, int LS = 0, class GK = RunGeom>
+template <int E, class DE, int LS = 0, class GK = RunGeom, bool TWR = false>
 __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
                                           const FacList &fl, const DictP &dict, int src_kind,
@@ -520,6 +520,22 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   if constexpr (kPre) {
     if (nf > 0) c2r_pre<E, LS, GK>(g, fl, b, twN, gl < nf ? gl : 0, r, t, T, v0);
   }
+  // With TWR (a compile-time line length, 8 values per thread) the twiddles
+  // of the passes are held in registers, loaded here in the same round trip
+  // (otherwise each pass reads them from the table in global memory; the
+  // inverse transforms conjugate at use).  The registers pay off where rows
+  // are many (3-D), not at 512^2: the tuner picks.
+  constexpr bool kTwR = TWR && LS > 0 && E == 8;
+  cx twr[kTwR ? static_tw_count<(LS > 0 ? LS : 8)>(E) : 1];
+  if constexpr (kTwR) load_tw_s<E, (LS > 0 ? LS : 8)>(twr, tw, t);
+  auto fft_c2r = [&](cx(&x)[E]) {
+    if constexpr (kTwR) reg_fft_w<E, +1, (LS > 0 ? LS : 8)>(x, lb, 1, twr, t);
+    else fft_any<E, +1, LS>(x, lb, 1, tw, px, t, T);
+  };
+  auto fft_r2c = [&](cx(&x)[E]) {
+    if constexpr (kTwR) reg_fft_w<E, -1, (LS > 0 ? LS : 8)>(x, lb, 1, twr, t);
+    else fft_any<E, -1, LS>(x, lb, 1, tw, px, t, T);
+  };
   Prefetch<4, int> pf_p;
   if constexpr (DE::kProg) pf_p.load(dict.prog, dict.nwords, tid, nt);
   constexpr int IDU = 8;
@@ -552,7 +568,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = cx{0.0, 0.0};
     }
-    fft_any<E, +1, LS>(v, lb, 1, tw, px, t, T);
+    fft_c2r(v);
     if (act) {
       double *vr = vals + (long)(n + m) * Nx;
 #pragma unroll
@@ -694,7 +710,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
     const double2 o = *reinterpret_cast<const double2 *>(ys + f * Nx + 2 * (t + T * q));
     v[q] = act ? cx{o.x, o.y} : cx{0.0, 0.0};
   }
-  fft_any<E, -1, LS>(v, lb, 1, tw, px, t, T);
+  fft_r2c(v);
 #pragma unroll
   for (int q = 0; q < E; ++q) lb[t + T * q] = v[q];
   __syncthreads();
@@ -721,7 +737,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
 // Block = R rows x G*T threads (R = blockDim / (G*T)), row r = blockIdx*R +
 // sub.  With 24 values per thread the per-row thread range and LDS offset
 // cost registers the kernel does not have, so those kernels take R = 1.
-template <int E, class DE, int LS = 0, class GK = RunGeom>
+template <int E, class DE, int LS = 0, class GK = RunGeom, bool TWR = false>
 __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
                                           const FacList &fl, const DictP &dict, int src_kind,
@@ -736,11 +752,11 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
     const int nf = GK::nf >= 0 ? GK::nf : fl.nf;
     const long region = rowi_region(L, Nx, n, nf, dict.nwords, G);
     const int gt = G * (L / E), sub = threadIdx.x / gt;
-    rowi_body<E, DE, LS, GK>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl,
+    rowi_body<E, DE, LS, GK, TWR>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl,
                              dict, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
                              smem + sub * region, threadIdx.x - sub * gt, gt, chk);
   } else {
-    rowi_body<E, DE, LS, GK>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
+    rowi_body<E, DE, LS, GK, TWR>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
                              dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x,
                              chk);
   }

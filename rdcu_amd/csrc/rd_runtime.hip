@@ -164,6 +164,8 @@ struct rdcu_ctx {
   // x-inverse kernel with the dictionary compiled in (hiprtc); null if not built
   hipModule_t jit_mod = nullptr;
   hipFunction_t jit_rowi = nullptr;
+  hipFunction_t jit_rowi_tw = nullptr;  // the same with the twiddles in registers
+  int ri_tw = 0;                        // the tuner's choice between the two
   // 1-D: a whole attempt in one workgroup (null if not built or not eligible)
   hipFunction_t jit_att = nullptr;
   size_t lds_att = 0;
@@ -501,8 +503,9 @@ void launch_rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, in
     int a_G = G;
     void *args[] = {&a_g, &a_px, &a_tw, &a_st, &a_check, &a_b, &a_fl, &a_d, &a_src,
                     &a_sp, &a_dk, &a_ds, &a_dp, &a_err, &a_ec, &a_nx, &a_G};
-    (void)hipModuleLaunchKernel(c->jit_rowi, grid, 1, 1, block, 1, 1, (unsigned)lds, c->stream,
-                                args, nullptr);
+    hipFunction_t fn = c->ri_tw && c->jit_rowi_tw ? c->jit_rowi_tw : c->jit_rowi;
+    (void)hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, (unsigned)lds, c->stream, args,
+                                nullptr);
     return;
   }
   hipLaunchKernelGGL(rd::k_rowi_reg<E>, dim3(grid), dim3(block), lds, c->stream, c->grid, c->px,
@@ -804,9 +807,19 @@ void build_jit(rdcu_ctx *c) {
            "Nxp = %d, nf = %d;\n  static constexpr long N = %ldL, rows = %ldL;\n};\n}\n",
            gd.n, gd.ndim, gd.Nx, gd.Nxp, c->fl_used.nf, gd.N, gd.rows);
   src = buf + src;
+  const std::string sig = src.substr(src.rfind("extern \"C\""));
   src += "  rd::rowi_rows<" + E + ", rd::JitDict, " + std::to_string(c->L) +
          ", rd::JitGeom>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot, "
          "dst_ptr, err, ec, nx, G, check_active ? st : nullptr);\n}\n";
+  // the same kernel with the transforms' twiddles in registers (the tuner
+  // times both)
+  if (c->Ex == 8) {
+    std::string tw = sig;
+    tw.replace(tw.find("rdcu_rowi_jit"), 13, "rdcu_rowi_jit_tw");
+    src += tw + "  rd::rowi_rows<" + E + ", rd::JitDict, " + std::to_string(c->L) +
+           ", rd::JitGeom, true>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, "
+           "dst_slot, dst_ptr, err, ec, nx, G, check_active ? st : nullptr);\n}\n";
+  }
   const size_t att_lds = attempt1d_lds(c);
   if (att_lds)
     src += "extern \"C\" __global__ __launch_bounds__(256) void rdcu_attempt1d_jit(rd::Grid g, "
@@ -825,6 +838,10 @@ void build_jit(rdcu_ctx *c) {
     c->jit_rowi = nullptr;
     return;
   }
+  if (c->Ex == 8 &&
+      (hipModuleGetFunction(&c->jit_rowi_tw, c->jit_mod, "rdcu_rowi_jit_tw") != hipSuccess ||
+       jit_spills(c->jit_rowi_tw)))
+    c->jit_rowi_tw = nullptr;
   if (att_lds && hipModuleGetFunction(&c->jit_att, c->jit_mod, "rdcu_attempt1d_jit") == hipSuccess &&
       !jit_spills(c->jit_att))
     c->lds_att = att_lds;
@@ -862,9 +879,9 @@ int pow2_above(int x) {
 }
 
 int fixed_shapes(rdcu_ctx *c, const char *spec) {
-  int rpb = 0, tc = 0, fpw = 0, G = 0, R = 1, Ec = 0, ntc = 0;
-  const int got = sscanf(spec, "%d,%d,%d,%d,%d,%d,%d", &rpb, &tc, &fpw, &G, &R, &Ec, &ntc);
-  if (got < 4) return fail(RDCU_EINVAL, "RDCU_SHAPES must be rpb,tc,fpw,G[,R[,Ecol[,ntc]]]");
+  int rpb = 0, tc = 0, fpw = 0, G = 0, R = 1, Ec = 0, ntc = 0, tw = 0;
+  const int got = sscanf(spec, "%d,%d,%d,%d,%d,%d,%d,%d", &rpb, &tc, &fpw, &G, &R, &Ec, &ntc, &tw);
+  if (got < 4) return fail(RDCU_EINVAL, "RDCU_SHAPES must be rpb,tc,fpw,G[,R[,Ecol[,ntc[,tw]]]]");
   if (c->ndim >= 2 && tc >= 1 && col_tiles(c, tc, ntc) == 0) ntc = 0;
   if (Ec == 16) {  // 16 values per thread on the power-of-two strided axes
     if (c->E0 == 8 && c->Ns[0] >= 16) c->E0 = 16;
@@ -894,6 +911,7 @@ int fixed_shapes(rdcu_ctx *c, const char *spec) {
   c->cr_fpw = fpw;
   c->ri_G = G;
   c->ri_R = R;
+  c->ri_tw = tw && c->jit_rowi_tw ? 1 : 0;
   c->tune_log = std::string("fixed ") + spec;
   return RDCU_OK;
 }
@@ -985,28 +1003,37 @@ int autotune(rdcu_ctx *c) {
     c->E1 = bestE1;
     c->cr_fpw = best_fpw;
   }
+  int best_tw = 0;
   // x inverse: factor lines per group, rows per block (dividing the rows).
   // Groups wider than the factor count (powers of two above it) leave lines
   // idle but spread the dictionary and the stage sums over more threads,
   // which pays when there are few rows (1-D: one row).
   best = 1e30;
   const int Gf = std::max(c->n, c->nf_used);
-  for (int G = c->n; G * T <= 256; G = G < Gf ? G + 1 : (G == Gf ? pow2_above(Gf) : 2 * G)) {
-    for (int R = 1; R * G * T <= 256 && (R == 1 || c->Ex <= 8); R *= 2) {
-      if (c->rows % R) break;
-      if (rowi_lds(c, G, R) > 160 * 1024) break;
-      double tm = time_of([&]() {
-        rowi(c, 0, rd::SRC_Y, nullptr, rd::DST_PTR, 0, c->rhs_out, 0, none, G, R);
-      });
-      snprintf(line, sizeof line, "row_inv G=%d R=%d %.2fus; ", G, R, tm * 1e3);
-      c->tune_log += line;
-      if (tm < best) {
-        best = tm;
-        c->ri_G = G;
-        c->ri_R = R;
+  // (with a generated kernel, also its variant with the twiddles in
+  // registers, timed on the fused pass of an attempt's stage, where it runs)
+  rd::StageCoef nx3 = stage_coef(3);
+  for (int tw = 0; tw <= (c->jit_rowi_tw ? 1 : 0); ++tw) {
+    c->ri_tw = tw;
+    for (int G = c->n; G * T <= 256; G = G < Gf ? G + 1 : (G == Gf ? pow2_above(Gf) : 2 * G)) {
+      for (int R = 1; R * G * T <= 256 && (R == 1 || c->Ex <= 8); R *= 2) {
+        if (c->rows % R) break;
+        if (rowi_lds(c, G, R) > 160 * 1024) break;
+        double tm = time_of([&]() {
+          rowi(c, 0, rd::SRC_Y, nullptr, rd::DST_KS, 1, nullptr, 0, nx3, G, R);
+        });
+        snprintf(line, sizeof line, "row_inv tw=%d G=%d R=%d %.2fus; ", tw, G, R, tm * 1e3);
+        c->tune_log += line;
+        if (tm < best) {
+          best = tm;
+          c->ri_G = G;
+          c->ri_R = R;
+          best_tw = tw;
+        }
       }
     }
   }
+  c->ri_tw = best_tw;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1525,9 +1552,9 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
     }
     const char *vb = getenv("RDCU_VERBOSE");
     if (vb && *vb && *vb != '0')
-      fprintf(stderr, "rdcu jit: %s\nrdcu autotune: %s\nrdcu shapes: %d,%d,%d,%d,%d,%d,%d\n",
+      fprintf(stderr, "rdcu jit: %s\nrdcu autotune: %s\nrdcu shapes: %d,%d,%d,%d,%d,%d,%d,%d\n",
               c->jit_log.c_str(), c->tune_log.c_str(), c->rf_rpb, c->cr_tc, c->cr_fpw, c->ri_G,
-              c->ri_R, c->E0 == 16 ? 16 : 8, c->cr_ntc);
+              c->ri_R, c->E0 == 16 ? 16 : 8, c->cr_ntc, c->ri_tw);
   }
 
   // capture one attempt
