@@ -92,6 +92,28 @@ def copy_calibration(mib, reps):
                       "ms": ms, "GBps": 2 * n * 8 / (ms * 1e-3) / 1e9}), flush=True)
 
 
+def batched_rhs(b, reps=20):
+    """One plain RHS of all instances of a batch (rdcu_batch_time: HIP events,
+    every instance in the launches): the time of the whole batched RHS, its
+    algorithmic bytes against the HBM peak, and per shape class the time, bytes
+    and bandwidth of each launch."""
+    ms, classes = b.time_rhs(reps)
+    total = sum(sum(c["bytes"].values()) for c in classes)
+    per = []
+    for c in classes:
+        per.append({"grid": c["grid"], "instances": c["instances"],
+                    "ms": c["ms"], "MB": {k: v / 1e6 for k, v in c["bytes"].items()},
+                    "GBps": {k: (c["bytes"][k] / (c["ms"][k] * 1e-3) / 1e9 if c["ms"][k] > 0
+                                 else None) for k in c["ms"]}})
+    out = {"ms": ms, "bytes": total, "achieved_GBps": total / (ms * 1e-3) / 1e9,
+           "frac": total / (ms * 1e-3) / PEAK_HBM, "reps": reps, "classes": per,
+           "what": "one plain RHS of every instance (all classes back to back), HIP events "
+                   "on the batch's stream; bytes: algorithmic (half spectra, one read and "
+                   "one write per array per pass)"}
+    log("batched rhs:", json.dumps(out))
+    return out
+
+
 def sweep_leg(lib, count, t_end, every):
     """Batched sweep against one-at-a-time plain solves on the same GPU.
 
@@ -116,6 +138,7 @@ def sweep_leg(lib, count, t_end, every):
     c2 = time.perf_counter()
     sub = list(range(0, n, every))
     yb = {i: b.member(i).state() for i in sub}
+    rhs = batched_rhs(b)  # after the run: not part of the timed window
     b.close()
     log(f"batched: {n} instances, setup {c1 - c0:.2f} s, run {c2 - c1:.3f} s, "
         f"attempts {int(na.sum())} (max {int(na.max())})")
@@ -156,6 +179,7 @@ def sweep_leg(lib, count, t_end, every):
         "serial_solves_per_s_with_setup": len(sub) / (setup_s + run_s),
         "batched_attempts": int(na.sum()), "serial_subset": len(sub),
         "identical_step_sequences": bool(same),
+        "rhs": rhs,
     }
     log("batched:", json.dumps(out))
     return out

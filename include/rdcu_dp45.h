@@ -217,6 +217,27 @@ int rdcu_batch_run(rdcu_batch *batch, double *t, double *h, double hmin, double 
  * rdcu_advance return RDCU_ESTATE for it). */
 int rdcu_batch_member(rdcu_batch *batch, int i, rdcu_ctx **out);
 
+/* Timing of one plain RHS of every instance, with HIP events on the batch's
+ * stream (a diagnostic, like rdcu_time_rhs; no counterpart in the reference,
+ * whose sweep runs one process per instance).  All instances, on their current
+ * states, with the batch's tuned launch shapes:
+ *   ms[0]: every shape class's x-forward, strided and x-inverse launches back to
+ *          back, averaged over `reps` evaluations;
+ *   ms[1+3k .. 3+3k]: class k's three launches, each alone, averaged over `reps`
+ *          (classes ordered by field count, dimension, then grid sizes).
+ * bytes[3k .. 3k+2] receive the algorithmic HBM bytes of class k's launches,
+ * summed over its instances (half spectra, one read and one write per array per
+ * pass: x forward 16nN; strided 8nN + 8MN; x inverse 8MN + 16nN, with M the
+ * instance's referenced factors; in 1-D no strided pass and 24nN for the x
+ * inverse).  shape[4k .. 4k+3] receives ndim, N0, N1 (1 in 1-D) and the
+ * instance count.  Up to max_classes classes are filled (ms needs
+ * 1 + 3*max_classes entries); *nclasses receives the number of classes.  Writes
+ * every instance's stage-2 buffer, which every attempt overwrites.  The CPU
+ * oracle fills ms[0] only (wall clock of the instances' RHS one after another)
+ * and sets *nclasses to 0. */
+int rdcu_batch_time(rdcu_batch *batch, int reps, double *ms, double *bytes, int *shape,
+                    int max_classes, int *nclasses);
+
 /* ---- Differentiation: gradients of an output-matching loss ---------------
  * No counterpart in the reference, whose only derivative code is the Jacobian
  * chain rule of its stiff path (rdcu.cu:204-216).  These serve fitting

@@ -1148,6 +1148,24 @@ int rdcu_batch_member(rdcu_batch *b, int i, rdcu_ctx **out) {
   return RDCU_OK;
 }
 
+int rdcu_batch_time(rdcu_batch *b, int reps, double *ms, double *bytes, int *shape,
+                    int max_classes, int *nclasses) {
+  (void)bytes;
+  (void)shape;
+  (void)max_classes;
+  if (!b || !ms || reps < 1) return fail(RDCU_EINVAL, "bad argument");
+  double t0 = omp_get_wtime();
+  for (int r = 0; r < reps; ++r)
+    for (rdcu_ctx *c : b->m) {
+      if (c->fn) return no_dict();
+      std::vector<double> f(c->y.size());
+      c->E.rhs(c->y.data(), f.data());
+    }
+  ms[0] = (omp_get_wtime() - t0) * 1e3 / reps;
+  if (nclasses) *nclasses = 0;
+  return RDCU_OK;
+}
+
 int rdcu_batch_step(rdcu_batch *b, double *t, double *h, double hmin, double hmax, int *accepted) {
   (void)hmin;
   if (!b || !t || !h || !accepted) return fail(RDCU_EINVAL, "null argument");

@@ -35,6 +35,7 @@ SYMBOLS = (
     "rdcu_batch_member", "rdcu_set_coef", "rdcu_schedule", "rdcu_loss_grad",
     "rdcu_loss_grad_batch", "rdcu_frame", "rdcu_host_alloc", "rdcu_host_free",
     "rdcu_rhs_noalloc", "rdcu_create_fn", "rdcu_rhs_device", "rdcu_loss_grad_device",
+    "rdcu_batch_time",
 )
 
 
@@ -491,6 +492,26 @@ class Batch:
 
     def member(self, i: int) -> Integrator:
         return _Member(self, i)
+
+    def time_rhs(self, reps=20, max_classes=64):
+        """One plain RHS of every instance, timed (rdcu_batch_time): returns
+        (ms of all classes back to back, [per class dict: ndim, grid,
+        instances, ms and algorithmic bytes of the x-forward, strided and
+        x-inverse launches])."""
+        ms = np.zeros(1 + 3 * max_classes)
+        by = np.zeros(3 * max_classes)
+        sh = (ctypes.c_int * (4 * max_classes))()
+        nc = ctypes.c_int(0)
+        self._check(self.lib.rdcu_batch_time(self.h, reps, _ptr(ms), _ptr(by), sh, max_classes,
+                                             ctypes.byref(nc)))
+        out = []
+        names = ("row_fwd", "col_mult", "row_inv")
+        for k in range(min(nc.value, max_classes)):
+            out.append({"ndim": sh[4 * k], "grid": [sh[4 * k + 1], sh[4 * k + 2]],
+                        "instances": sh[4 * k + 3],
+                        "ms": {nm: float(ms[1 + 3 * k + j]) for j, nm in enumerate(names)},
+                        "bytes": {nm: float(by[3 * k + j]) for j, nm in enumerate(names)}})
+        return float(ms[0]), out
 
     def step(self, t, h, hmin=0.0, hmax=np.inf):
         """One attempt of every instance: returns (t, h, accepted) arrays."""

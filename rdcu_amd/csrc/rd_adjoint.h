@@ -44,7 +44,7 @@ struct ProgVjp {
                                              double *, double *, int, int, int, int) {}
 };
 
-template <int E, class VD>
+template <int E, class VD, int LS = 0>
 __device__ __forceinline__ void rowi_vjp_body(const Grid &g, const FftPlan &px,
                                               const cx *__restrict__ twN, const Bufs &b,
                                               const FacList &fl, const DictP &dict,
@@ -88,7 +88,7 @@ __device__ __forceinline__ void rowi_vjp_body(const Grid &g, const FftPlan &px,
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = cx{0.0, 0.0};
     }
-    reg_fft<E, +1>(v, lb, 1, tw, px, t, T);
+    fft_any<E, +1, LS>(v, lb, 1, tw, px, t, T);
     if (act) {
       double *vr = vals + (long)(n + m) * Nx;
 #pragma unroll
@@ -161,7 +161,7 @@ __device__ __forceinline__ void rowi_vjp_body(const Grid &g, const FftPlan &px,
   // field m is weight row m), with the operations of rowf_body: the real row
   // as L complex values, forward transform, then the split into the
   // half spectrum, written to the companion's S.  Only with 8 values per
-  // thread: the 24-value kernel has no registers left for it (Spc is null).
+  // thread: kernels with more have no registers left for it (Spc is null).
   if constexpr (E != 8) return;
   if (!Spc) return;
   const long Nxp = g.Nxp;
@@ -176,7 +176,7 @@ __device__ __forceinline__ void rowi_vjp_body(const Grid &g, const FftPlan &px,
       const double2 o = *reinterpret_cast<const double2 *>(wr + 2 * (t + T * q));
       v[q] = act ? cx{o.x, o.y} : cx{0.0, 0.0};
     }
-    reg_fft<E, -1>(v, lb, 1, tw, px, t, T);
+    fft_any<E, -1, LS>(v, lb, 1, tw, px, t, T);
 #pragma unroll
     for (int q = 0; q < E; ++q) lb[t + T * q] = v[q];
     __syncthreads();
@@ -202,14 +202,15 @@ __device__ __forceinline__ void rowi_vjp_body(const Grid &g, const FftPlan &px,
   }
 }
 
-template <int E>
+// LS: 0 for power-of-two lines, -1 for 3*2^k lines (run-time plans)
+template <int E, int LS = 0>
 __global__ __launch_bounds__(256) void k_rowi_vjp(Grid g, FftPlan px, const cx *__restrict__ twN,
                                                   Bufs b, FacList fl, DictP dict,
                                                   const int *__restrict__ tmap,
                                                   const double *__restrict__ u,
                                                   const double *__restrict__ W, double *out_id,
                                                   double *Gk, double *P, int G, cx *Spc) {
-  rowi_vjp_body<E, ProgVjp>(g, px, twN, b, fl, dict, tmap, u, W, out_id, Gk, P, G, Spc);
+  rowi_vjp_body<E, ProgVjp, LS>(g, px, twN, b, fl, dict, tmap, u, W, out_id, Gk, P, G, Spc);
 }
 
 // The seven stage values of the step just taken, into dst[j*nN + i]: every

@@ -168,7 +168,8 @@ void build_jit_vjp(rdcu_ctx *c) {
          "rd::FftPlan px, const rd::cx *__restrict__ twN, rd::Bufs b, rd::FacList fl, rd::DictP dict, "
          "const int *__restrict__ tmap, const double *__restrict__ u, const double *__restrict__ W, "
          "double *out_id, double *Gk, double *P, int G, rd::cx *Spc) {\n"
-         "  rd::rowi_vjp_body<" + std::to_string(c->Ex) + ", rd::JitVjp>(g, px, twN, b, fl, dict, "
+         "  rd::rowi_vjp_body<" + std::to_string(c->Ex) + ", rd::JitVjp, " +
+         (odd_len(c->L) ? "-1" : "0") + ">(g, px, twN, b, fl, dict, "
          "tmap, u, W, out_id, Gk, P, G, Spc);\n}\n";
   std::vector<char> code;
   std::string log;
@@ -272,12 +273,12 @@ int vjp(rdcu_ctx *c, const double *u, const double *w, double *out_id, double *G
     void *args[] = {&a_g, &a_px, &a_tw, &a_b, &a_fl, &a_d, &a_tm, &a_u, &a_w, &a_oid, &a_gk, &a_p, &a_G, &a_s};
     HIPCHECK(hipModuleLaunchKernel(c->jit_vjp, (unsigned)c->rows, 1, 1, (unsigned)nt, 1, 1,
                                    (unsigned)lds, c->stream, args, nullptr));
-  } else if (c->Ex == 8)
-    hipLaunchKernelGGL(rd::k_rowi_vjp<8>, dim3((unsigned)c->rows), dim3(nt), lds, c->stream,
+  } else if (odd_len(c->L))
+    hipLaunchKernelGGL((rd::k_rowi_vjp<8, -1>), dim3((unsigned)c->rows), dim3(nt), lds, c->stream,
                        c->grid, c->px, c->twN, bv, c->fl_used, c->dictp, tmap, u, w, out_id,
                        Gk, P, G, spc);
   else
-    hipLaunchKernelGGL(rd::k_rowi_vjp<24>, dim3((unsigned)c->rows), dim3(nt), lds, c->stream,
+    hipLaunchKernelGGL(rd::k_rowi_vjp<8>, dim3((unsigned)c->rows), dim3(nt), lds, c->stream,
                        c->grid, c->px, c->twN, bv, c->fl_used, c->dictp, tmap, u, w, out_id,
                        Gk, P, G, spc);
   HIPCHECK(hipGetLastError());

@@ -116,6 +116,7 @@ void launch_rowf_b(rdcu_batch *b, const BatchClass &cl, int src_kind, rd::StageC
     if (c->L == kRowLsB[0]) return go_rowf_b<E, kRowLsB[0]>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
     if (c->L == kRowLsB[1]) return go_rowf_b<E, kRowLsB[1]>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
     if (c->L == kRowLsB[2]) return go_rowf_b<E, kRowLsB[2]>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
+    if (odd_len(c->L)) return go_rowf_b<E, -1>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
   }
   go_rowf_b<E, 0>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
 }
@@ -144,6 +145,7 @@ void launch_col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, int chec
     if (cg.L == kColLsB[0]) return go_col_b<E, kColLsB[0]>(b, cl, gd, bl, lds, per, cg, fpw, check);
     if (cg.L == kColLsB[1]) return go_col_b<E, kColLsB[1]>(b, cl, gd, bl, lds, per, cg, fpw, check);
     if (cg.L == kColLsB[2]) return go_col_b<E, kColLsB[2]>(b, cl, gd, bl, lds, per, cg, fpw, check);
+    if (odd_len(cg.L)) return go_col_b<E, -1>(b, cl, gd, bl, lds, per, cg, fpw, check);
   }
   go_col_b<E, 0>(b, cl, gd, bl, lds, per, cg, fpw, check);
 }
@@ -171,6 +173,10 @@ void launch_rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_ki
                       &a_chk, &region};
       (void)hipModuleLaunchKernel(gr.fn, grid, 1, 1, block, 1, 1, (unsigned)lds, b->stream, args,
                                   nullptr);
+    } else if (odd_len(c->L)) {
+      hipLaunchKernelGGL((rd::k_rowi_b<E, -1>), dim3(grid), dim3(block), lds, b->stream, in, cnt,
+                         rows, c->px, c->twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check,
+                         region);
     } else {
       hipLaunchKernelGGL(rd::k_rowi_b<E>, dim3(grid), dim3(block), lds, b->stream, in, cnt, rows,
                          c->px, c->twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check,
@@ -181,18 +187,15 @@ void launch_rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_ki
 
 void rowf_b(rdcu_batch *b, const BatchClass &cl, int src_kind, rd::StageCoef sc, int rpb,
             long grid, int check) {
-  if (cl.Ex == 8) launch_rowf_b<8>(b, cl, src_kind, sc, rpb, grid, check);
-  else launch_rowf_b<24>(b, cl, src_kind, sc, rpb, grid, check);
+  launch_rowf_b<8>(b, cl, src_kind, sc, rpb, grid, check);
 }
 void col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, int check) {
   if (cl.c0->ndim < 2) return;
-  if (cl.E0 == 8) launch_col_b<8>(b, cl, tc, fpw, check);
-  else launch_col_b<24>(b, cl, tc, fpw, check);
+  launch_col_b<8>(b, cl, tc, fpw, check);
 }
 void rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_kind, int dst_slot, int err,
             rd::StageCoef nx, int G, int R, int check) {
-  if (cl.Ex == 8) launch_rowi_b<8>(b, cl, src_kind, dst_kind, dst_slot, err, nx, G, R, check);
-  else launch_rowi_b<24>(b, cl, src_kind, dst_kind, dst_slot, err, nx, G, R, check);
+  launch_rowi_b<8>(b, cl, src_kind, dst_kind, dst_slot, err, nx, G, R, check);
 }
 
 // One plain RHS of every instance of a class (input y, output k slot `slot`
@@ -590,9 +593,9 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
     if (!lds_raised) {
       const int lim = 160 * 1024;
       for (const void *f :
-           {(const void *)rd::k_rowf_b<8>, (const void *)rd::k_rowf_b<24>,
-            (const void *)rd::k_col_b<8>, (const void *)rd::k_col_b<24>,
-            (const void *)rd::k_rowi_b<8>, (const void *)rd::k_rowi_b<24>,
+           {(const void *)rd::k_rowf_b<8>, (const void *)rd::k_rowf_b<8, -1>,
+            (const void *)rd::k_col_b<8>, (const void *)rd::k_col_b<8, -1>,
+            (const void *)rd::k_rowi_b<8>, (const void *)rd::k_rowi_b<8, -1>,
             (const void *)rd::k_rowf_b<8, kRowLsB[0]>, (const void *)rd::k_rowf_b<8, kRowLsB[1]>,
             (const void *)rd::k_rowf_b<8, kRowLsB[2]>, (const void *)rd::k_col_b<8, kColLsB[0]>,
             (const void *)rd::k_col_b<8, kColLsB[1]>, (const void *)rd::k_col_b<8, kColLsB[2]>})
@@ -656,6 +659,74 @@ int rdcu_batch_member(rdcu_batch *b, int i, rdcu_ctx **out) {
   if (!b || !out) return fail(RDCU_EINVAL, "null argument");
   if (i < 0 || i >= ninst(b)) return fail(RDCU_EINVAL, "instance index out of range");
   *out = b->members[i].get();
+  return RDCU_OK;
+}
+
+int rdcu_batch_time(rdcu_batch *b, int reps, double *ms, double *bytes, int *shape,
+                    int max_classes, int *nclasses) {
+  if (!b || !ms || reps < 1 || max_classes < 0) return fail(RDCU_EINVAL, "bad argument");
+  HIPCHECK(hipSetDevice(b->dev));
+  int st;
+  if ((st = full_table(b))) return st;  // every instance in the table
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  auto timed = [&](const std::function<void()> &body, double *out) -> int {
+    for (int r = 0; r < 2; ++r) body();  // warm
+    HIPCHECK(hipEventRecord(e0, b->stream));
+    for (int r = 0; r < reps; ++r) body();
+    HIPCHECK(hipEventRecord(e1, b->stream));
+    HIPCHECK(hipEventSynchronize(e1));
+    float f = 0;
+    HIPCHECK(hipEventElapsedTime(&f, e0, e1));
+    *out = (double)f / reps;
+    return RDCU_OK;
+  };
+  rd::StageCoef none{};
+  // the plain RHS into the stage-2 slot (k2), as the tuner does
+  st = timed([&]() {
+    for (const BatchClass &cl : b->classes) rhs_b(b, cl, rd::DST_KS, 1, 0);
+  }, &ms[0]);
+  const int nc = (int)b->classes.size();
+  for (int k = 0; !st && k < nc && k < max_classes; ++k) {
+    const BatchClass &cl = b->classes[k];
+    const rdcu_ctx *c0 = cl.c0;
+    double *o = ms + 1 + 3 * k;
+    st = timed([&]() { rowf_b(b, cl, rd::SRC_Y, none, cl.rpb, cl.grid_rf, 0); }, &o[0]);
+    if (!st && c0->ndim >= 2) st = timed([&]() { col_b(b, cl, cl.tc, cl.fpw, 0); }, &o[1]);
+    else o[1] = 0.0;
+    if (!st)
+      st = timed([&]() { rowi_b(b, cl, rd::SRC_Y, rd::DST_KS, 1, 0, none, cl.G, cl.R, 0); },
+                 &o[2]);
+    if (bytes) {
+      double bf = 0, bc = 0, bi = 0;
+      for (int i : cl.members) {
+        const rdcu_ctx *m = b->members[i].get();
+        const double nN = (double)m->n * m->N, MN = (double)m->nf_used * m->N;
+        bf += 16 * nN;
+        if (m->ndim >= 2) {
+          bc += 8 * nN + 8 * MN;
+          bi += 8 * MN + 16 * nN;
+        } else {
+          bi += 24 * nN;
+        }
+      }
+      bytes[3 * k] = bf;
+      bytes[3 * k + 1] = bc;
+      bytes[3 * k + 2] = bi;
+    }
+    if (shape) {
+      shape[4 * k] = c0->ndim;
+      shape[4 * k + 1] = c0->Ns[0];
+      shape[4 * k + 2] = c0->ndim >= 2 ? c0->Ns[1] : 1;
+      shape[4 * k + 3] = (int)cl.members.size();
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  HIPCHECK(hipGetLastError());
+  if (nclasses) *nclasses = nc;
   return RDCU_OK;
 }
 

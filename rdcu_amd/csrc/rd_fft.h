@@ -197,7 +197,8 @@ __device__ cx *lds_fft(cx *a, cx *b, const FftPlan &p, int nl, int ls, int es, i
 // own registers -- then the same twiddle/butterfly arithmetic as
 // stockham_pass, outputs to LDS at (j-k)*R + k + r*Ns.  The outputs of the
 // last pass land at j + r*Ns = the thread's own slots, so the result stays in
-// registers.  E must be a multiple of every radix of the plan.
+// registers.  A radix that does not divide E (the leading radix-3 pass of a
+// length 3*2^k with E = 8) runs through the line in LDS instead (lds_pass).
 // LDS slot of element e in the register path: one pad slot per 8 elements,
 // so the stride-R scatter of the early passes spreads over all banks.
 __device__ __forceinline__ int pad8(int e) { return e + (e >> 3); }
@@ -235,14 +236,69 @@ __device__ __forceinline__ void reg_pass(cx (&v)[E], cx *buf, int es, const cx *
   }
 }
 
+// A pass whose radix does not divide the values per thread (the radix-3 and
+// radix-5 passes that lead the plan of a length 3^b 5^c 2^a): butterfly
+// j = t + T*b (b < ceil(E/R), j < L/R) reads its inputs from the line in
+// LDS, and after a barrier writes its outputs back, with the twiddle and
+// butterfly arithmetic of reg_pass (so the values are the same).  With
+// from_regs (the transform's first pass) the thread's values t + T*q are
+// stored to the line first.  Never the last pass of a plan (those end in
+// radix 2, 4 or 8), so the next pass reads its values back from the line.
+template <int R, int E, int DIR>
+__device__ __forceinline__ void lds_pass(cx (&v)[E], cx *buf, int es, const cx *tw, int L, int Ns,
+                                         int t, int T, bool from_regs) {
+  constexpr int NB = (E + R - 1) / R;
+  if (from_regs) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) buf[pad8(t + T * q) * es] = v[q];
+  }
+  __syncthreads();
+  const int nbf = L / R, twstep = L / (Ns * R);
+  cx u[NB][R];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + T * b;
+    if (j < nbf) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) u[b][r] = buf[pad8(j + nbf * r) * es];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + T * b;
+    if (j < nbf) {
+      const int k = j % Ns;
+      if (k != 0) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cx w = tw[(k * r) * twstep];
+          if constexpr (DIR > 0) w.im = -w.im;
+          u[b][r] = cmul(u[b][r], w);
+        }
+      }
+      dft<R, DIR>(u[b]);
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[pad8(base + r * Ns) * es] = u[b][r];
+    }
+  }
+}
+
 // Full transform of the line whose element e lives at buf[pad8(e)*es] (LDS
 // scratch of padded_len(L) elements); input and output in v.  Every thread of the block calls it
-// (it contains barriers); on return the scratch may be reused.
-template <int E, int DIR>
+// (it contains barriers); on return the scratch may be reused.  With ODD
+// (lengths 3*2^k), radix-3 passes run through the line (lds_pass); a
+// separate instantiation, so that kernels for powers of two carry none of
+// its registers.
+template <int E, int DIR, bool ODD = false>
 __device__ void reg_fft(cx (&v)[E], cx *buf, int es, const cx *tw, const FftPlan &p, int t, int T) {
   int Ns = 1;
   for (int ip = 0; ip < p.np; ++ip) {
     const int R = p.rad(ip);
+    if constexpr (ODD && E % 3 != 0) {
+      if (R == 3) { lds_pass<3, E, DIR>(v, buf, es, tw, p.L, Ns, t, T, ip == 0); Ns *= R; continue; }
+    }
     if (ip > 0) {
       __syncthreads();
 #pragma unroll
@@ -330,15 +386,27 @@ __device__ __forceinline__ void reg_fft_s(cx (&v)[E], cx *buf, int es, const cx 
   if constexpr (IP < NP) {
     constexpr int R = static_plan_rad(L, IP);
     constexpr int T = L / E;
-    if constexpr (IP > 0) {
-      __syncthreads();
+    if constexpr (E % R != 0) {  // through the line (lds_pass); never the last pass
+      static_assert(IP < NP - 1, "a plan ends in a radix that divides the values per thread");
+      lds_pass<R, E, DIR>(v, buf, es, tw, L, NS, t, T, IP == 0);
+    } else {
+      if constexpr (IP > 0) {
+        __syncthreads();
 #pragma unroll
-      for (int q = 0; q < E; ++q) v[q] = buf[pad8(t + T * q) * es];
-      __syncthreads();
+        for (int q = 0; q < E; ++q) v[q] = buf[pad8(t + T * q) * es];
+        __syncthreads();
+      }
+      reg_pass_s<R, E, DIR, L, NS, IP == NP - 1>(v, buf, es, tw, t);
     }
-    reg_pass_s<R, E, DIR, L, NS, IP == NP - 1>(v, buf, es, tw, t);
     reg_fft_s<E, DIR, L, IP + 1, NS * R>(v, buf, es, tw, t);
   }
+}
+
+// Whether every radix of the plan of L divides E (all passes in registers).
+constexpr bool static_plan_in_regs(int L, int E) {
+  for (unsigned long long c = static_plan_code(L); c; c >>= 4)
+    if (E % (int)(c & 15u) != 0) return false;
+  return true;
 }
 
 // The same compile-time transform with the twiddles in registers: every
@@ -424,12 +492,13 @@ __device__ __forceinline__ void reg_fft_w(cx (&v)[E], cx *buf, int es, const cx 
 }
 
 // reg_fft with the length fixed at compile time when LS > 0 (LS must then
-// equal p.L), the run-time plan otherwise.
+// equal p.L), the run-time plan otherwise: LS = 0 for powers of two, LS = -1
+// for lengths 3*2^k (radix-3 passes through the line).
 template <int E, int DIR, int LS>
 __device__ __forceinline__ void fft_any(cx (&v)[E], cx *buf, int es, const cx *tw, const FftPlan &p,
                                         int t, int T) {
   if constexpr (LS > 0) reg_fft_s<E, DIR, LS>(v, buf, es, tw, t);
-  else reg_fft<E, DIR>(v, buf, es, tw, p, t, T);
+  else reg_fft<E, DIR, (LS < 0)>(v, buf, es, tw, p, t, T);
 }
 
 }  // namespace rd

@@ -389,14 +389,15 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   }
 }
 
-// Up to 512 threads per block with 8 values per thread (204 registers fit
-// two waves per SIMD); 16 and 24 values per thread need more than 256
-// registers, so their blocks stay at 256 threads and nothing spills.
-template <int E>
-constexpr int col_max_threads() { return E == 8 ? 512 : 256; }
+// Up to 512 threads per block with 8 values per thread on power-of-two
+// lines (204 registers fit two waves per SIMD); 16 values per thread, and 3*2^k
+// lines (their radix-3 pass through LDS), need more than 256 registers, so
+// their blocks stay at 256 threads and nothing spills.
+template <int E, int LS = 0>
+constexpr int col_max_threads() { return E == 8 && LS >= 0 ? 512 : 256; }
 
 template <int E, bool NT, int LS = 0, bool TG = false>
-__global__ __launch_bounds__(col_max_threads<E>()) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
+__global__ __launch_bounds__((col_max_threads<E, LS>())) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
                                                  int check_active, Bufs b, FacList fl, int mode,
                                                  int fpw) {
   const int lblk = xcd_block(blockIdx.x, gridDim.x);
@@ -525,7 +526,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   // (otherwise each pass reads them from the table in global memory; the
   // inverse transforms conjugate at use).  The registers pay off where rows
   // are many (3-D), not at 512^2: the tuner picks.
-  constexpr bool kTwR = TWR && LS > 0 && E == 8;
+  constexpr bool kTwR = TWR && LS > 0 && E == 8 && static_plan_in_regs(LS > 0 ? LS : 8, E);
   cx twr[kTwR ? static_tw_count<(LS > 0 ? LS : 8)>(E) : 1];
   if constexpr (kTwR) load_tw_s<E, (LS > 0 ? LS : 8)>(twr, tw, t);
   auto fft_c2r = [&](cx(&x)[E]) {
@@ -735,8 +736,8 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
 }
 
 // Block = R rows x G*T threads (R = blockDim / (G*T)), row r = blockIdx*R +
-// sub.  With 24 values per thread the per-row thread range and LDS offset
-// cost registers the kernel does not have, so those kernels take R = 1.
+// sub.  With more than 8 values per thread the per-row thread range and LDS
+// offset would cost registers such kernels do not have, so they take R = 1.
 template <int E, class DE, int LS = 0, class GK = RunGeom, bool TWR = false>
 __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
@@ -794,15 +795,16 @@ __device__ __forceinline__ void attempt1d_body(const Grid &g, const FftPlan &px,
   control_step(st, b.part, (long)g.n * g.rows, (long)g.n * g.N, reinterpret_cast<double *>(smem));
 }
 
-template <int E>
+// LS: 0 for power-of-two lines, -1 for 3*2^k lines (run-time plans)
+template <int E, int LS = 0>
 __global__ __launch_bounds__(256) void k_rowi_reg(Grid g, FftPlan px, const cx *__restrict__ twN,
                                                   DevState *st, int check_active, Bufs b,
                                                   FacList fl, DictP dict, int src_kind,
                                                   const double *src_ptr, int dst_kind, int dst_slot,
                                                   double *dst_ptr, int err, ErrCoef ec,
                                                   StageCoef nx, int G) {
-  rowi_rows<E, ProgDict>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot,
-                         dst_ptr, err, ec, nx, G, check_active ? st : nullptr);
+  rowi_rows<E, ProgDict, LS>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot,
+                             dst_ptr, err, ec, nx, G, check_active ? st : nullptr);
 }
 
 // ---------------------------------------------------------------- batches
@@ -829,7 +831,7 @@ __global__ __launch_bounds__(256) void k_rowf_b(const Inst *__restrict__ in, int
 }
 
 // One (instance, column tile) item per block (a grid-stride loop around the
-// column pass spills at 24 elements per thread).  The grid is a multiple of
+// column pass costs registers it does not have).  The grid is a multiple of
 // 8 and items are ordered so that each XCD takes a contiguous range of
 // logical blocks: neighbouring column tiles of one instance share its L2.
 template <int E, int LS = 0>
@@ -846,7 +848,7 @@ __global__ __launch_bounds__(256) void k_col_b(const Inst *__restrict__ in, int 
 
 // The x-inverse pass takes one (instance, row) item per block: it is the
 // kernel with the most live registers, and a grid-stride loop around it
-// spills at 24 elements per thread.
+// would cost more.
 template <int E, class DE, int LS = 0, class GK = RunGeom>
 __device__ __forceinline__ void rowi_batch(const Inst *__restrict__ in, int ninst, int rows,
                                            const FftPlan &px, const cx *__restrict__ twN,
@@ -879,14 +881,14 @@ __device__ __forceinline__ void rowi_batch(const Inst *__restrict__ in, int nins
   }
 }
 
-template <int E>
+template <int E, int LS = 0>
 __global__ __launch_bounds__(256) void k_rowi_b(const Inst *__restrict__ in, int ninst, int rows,
                                                 FftPlan px, const cx *__restrict__ twN,
                                                 int src_kind, int dst_kind, int dst_slot, int err,
                                                 ErrCoef ec, StageCoef nx, int G, int check,
                                                 long region) {
-  rowi_batch<E, ProgDict>(in, ninst, rows, px, twN, src_kind, dst_kind, dst_slot, err, ec, nx,
-                          G, check, region);
+  rowi_batch<E, ProgDict, LS>(in, ninst, rows, px, twN, src_kind, dst_kind, dst_slot, err, ec, nx,
+                              G, check, region);
 }
 
 }  // namespace rd

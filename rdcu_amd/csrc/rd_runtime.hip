@@ -371,11 +371,12 @@ int enqueue_rhs(rdcu_ctx *c, int stage, int src_kind, const double *src_ptr, int
 // ---- register-resident path ------------------------------------------------
 
 // Elements per thread of the register Stockham for a length (0 = use the
-// LDS path): 8 for powers of two >= 8, 24 for 3*2^k >= 24.
+// LDS path): 8 for powers of two >= 8 and for 3*2^k >= 24 (whose leading
+// radix-3 pass runs through the line in LDS, lds_pass).
 int reg_elems(int L) {
   auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
   if (L >= 8 && pow2(L)) return 8;
-  if (L >= 24 && L % 3 == 0 && pow2(L / 3)) return 24;
+  if (L >= 24 && L % 3 == 0 && pow2(L / 3)) return 8;
   return 0;
 }
 
@@ -393,6 +394,10 @@ void go_rowf(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, int check, int src_kind,
                      c->st, check, c->bufs, src_kind, src_ptr, sc, rpb);
 }
 
+// Lengths 3*2^k run the run-time-plan kernels instantiated with LS = -1
+// (their radix-3 pass through LDS); powers of two those with LS = 0.
+bool odd_len(int L) { return L % 3 == 0; }
+
 template <int E, bool NT>
 void go_rowf_l(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, int check, int src_kind,
                const double *src_ptr, const rd::StageCoef &sc, int rpb) {
@@ -400,6 +405,7 @@ void go_rowf_l(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, int check, int src_kind,
     if (c->L == kRowLs[0]) return go_rowf<E, NT, kRowLs[0]>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
     if (c->L == kRowLs[1]) return go_rowf<E, NT, kRowLs[1]>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
   }
+  if (odd_len(c->L)) return go_rowf<E, NT, -1>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
   go_rowf<E, NT, 0>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
 }
 
@@ -427,7 +433,7 @@ void go_col(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, const rd::FftPlan &p, const
 // twiddles and wavenumbers from global memory (k_col_reg<8, NT, 0, true>):
 // the LDS copies would leave room for a single column per block.
 constexpr int kColTgMin = 1024;
-bool col_tg(int L, int E) { return E == 8 && L >= kColTgMin; }
+bool col_tg(int L, int E) { return E == 8 && L >= kColTgMin && !odd_len(L); }
 
 // LDS of a strided-pass block of tc columns of length L (mode 1: with the
 // wavenumbers, and with the kept spectrum when a block takes several
@@ -451,6 +457,7 @@ void go_col_l(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, const rd::FftPlan &p, con
                          c->st, check, c->bufs, fl, mode, fpw);
       return;
     }
+    if (odd_len(cg.L)) return go_col<E, NT, -1>(c, g, b, lds, p, cg, check, fl, mode, fpw);
   }
   go_col<E, NT, 0>(c, g, b, lds, p, cg, check, fl, mode, fpw);
 }
@@ -508,27 +515,27 @@ void launch_rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, in
                                 nullptr);
     return;
   }
-  hipLaunchKernelGGL(rd::k_rowi_reg<E>, dim3(grid), dim3(block), lds, c->stream, c->grid, c->px,
-                     c->twN, c->st, check, c->bufs, c->fl_used, c->dictp, src_kind, src_ptr,
-                     dst_kind, dst_slot, dst_ptr, err, ec, nx, G);
+  if (odd_len(c->L))
+    hipLaunchKernelGGL((rd::k_rowi_reg<E, -1>), dim3(grid), dim3(block), lds, c->stream, c->grid,
+                       c->px, c->twN, c->st, check, c->bufs, c->fl_used, c->dictp, src_kind,
+                       src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G);
+  else
+    hipLaunchKernelGGL(rd::k_rowi_reg<E>, dim3(grid), dim3(block), lds, c->stream, c->grid, c->px,
+                       c->twN, c->st, check, c->bufs, c->fl_used, c->dictp, src_kind, src_ptr,
+                       dst_kind, dst_slot, dst_ptr, err, ec, nx, G);
 }
 
 void rowf(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, rd::StageCoef sc, int rpb) {
-  if (c->Ex == 8) launch_rowf<8>(c, check, src_kind, src_ptr, sc, rpb);
-  else launch_rowf<24>(c, check, src_kind, src_ptr, sc, rpb);
+  launch_rowf<8>(c, check, src_kind, src_ptr, sc, rpb);
 }
 void colr(rdcu_ctx *c, const rd::FftPlan &p, int E, rd::ColGeom cg, long ngroups, int check,
           const rd::FacList &fl, int mode) {
-  if (E == 8) launch_colr<8>(c, p, cg, ngroups, check, fl, mode);
-  else if (E == 16) launch_colr<16>(c, p, cg, ngroups, check, fl, mode);
-  else launch_colr<24>(c, p, cg, ngroups, check, fl, mode);
+  if (E == 16) launch_colr<16>(c, p, cg, ngroups, check, fl, mode);
+  else launch_colr<8>(c, p, cg, ngroups, check, fl, mode);
 }
 void rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, int dst_kind, int dst_slot,
           double *dst_ptr, int err, rd::StageCoef nx, int G, int R) {
-  if (c->Ex == 8)
-    launch_rowi<8>(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, nx, G, R);
-  else
-    launch_rowi<24>(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, nx, G, R);
+  launch_rowi<8>(c, check, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, nx, G, R);
 }
 
 rd::StageCoef stage_coef(int stage) {
@@ -772,7 +779,7 @@ bool jit_spills(hipFunction_t f) {
 }
 
 // LDS of the single-workgroup 1-D attempt (256 threads: G = 256/T line
-// groups), or 0 where it does not apply (not 1-D, 24 values per thread, fewer
+// groups), or 0 where it does not apply (not 1-D, not the register path, fewer
 // line groups than fields, or more LDS than a CU has).
 size_t attempt1d_lds(const rdcu_ctx *c) {
   if (c->ndim != 1 || !c->reg || c->Ex != 8) return 0;
@@ -868,8 +875,12 @@ bool col_fits(const rdcu_ctx *c, int tc, int fpw) {
 // Largest strided-pass block for the values per thread in use (the
 // kernels' launch bounds).
 int col_threads(const rdcu_ctx *c) {
-  const bool e8 = c->E0 == 8 && (c->ndim < 3 || c->E1 == 8);
-  return e8 ? rd::col_max_threads<8>() : rd::col_max_threads<16>();
+  auto lim = [](int L, int E) {
+    return E == 8 && !odd_len(L) ? rd::col_max_threads<8>() : rd::col_max_threads<16>();
+  };
+  int m = lim(c->Ns[0], c->E0);
+  if (c->ndim == 3) m = std::min(m, lim(c->Ns[1], c->E1));
+  return m;
 }
 
 int pow2_above(int x) {
@@ -884,8 +895,8 @@ int fixed_shapes(rdcu_ctx *c, const char *spec) {
   if (got < 4) return fail(RDCU_EINVAL, "RDCU_SHAPES must be rpb,tc,fpw,G[,R[,Ecol[,ntc[,tw]]]]");
   if (c->ndim >= 2 && tc >= 1 && col_tiles(c, tc, ntc) == 0) ntc = 0;
   if (Ec == 16) {  // 16 values per thread on the power-of-two strided axes
-    if (c->E0 == 8 && c->Ns[0] >= 16) c->E0 = 16;
-    if (c->ndim == 3 && c->E1 == 8 && c->Ns[1] >= 16) c->E1 = 16;
+    if (c->E0 == 8 && c->Ns[0] >= 16 && !odd_len(c->Ns[0])) c->E0 = 16;
+    if (c->ndim == 3 && c->E1 == 8 && c->Ns[1] >= 16 && !odd_len(c->Ns[1])) c->E1 = 16;
   } else if (Ec != 0 && Ec != 8) {
     return fail(RDCU_EINVAL, "RDCU_SHAPES column values per thread must be 8 or 16");
   }
@@ -957,8 +968,9 @@ int autotune(rdcu_ctx *c) {
     for (int ev = 0; ev < 2; ++ev) {
       if (ev == 1) {
         bool any = false;
-        if (E0b == 8 && c->Ns[0] >= 16) c->E0 = 16, any = true;
-        if (c->ndim == 3 && E1b == 8 && c->Ns[1] >= 16) c->E1 = 16, any = true;
+        if (E0b == 8 && c->Ns[0] >= 16 && !odd_len(c->Ns[0])) c->E0 = 16, any = true;
+        if (c->ndim == 3 && E1b == 8 && c->Ns[1] >= 16 && !odd_len(c->Ns[1]))
+          c->E1 = 16, any = true;
         if (!any) break;
       }
       int Lmax = c->Ns[0], Tmax = c->Ns[0] / c->E0;
@@ -1046,13 +1058,13 @@ int autotune(rdcu_ctx *c) {
 int allow_big_lds() {
   const int lim = 160 * 1024;
   const void *fs[] = {
-      (const void *)rd::k_rowf_reg<8, false>,  (const void *)rd::k_rowf_reg<8, true>,
-      (const void *)rd::k_rowf_reg<24, false>, (const void *)rd::k_rowf_reg<24, true>,
-      (const void *)rd::k_col_reg<8, false>,   (const void *)rd::k_col_reg<8, true>,
-      (const void *)rd::k_col_reg<16, false>,  (const void *)rd::k_col_reg<16, true>,
-      (const void *)rd::k_col_reg<24, false>,  (const void *)rd::k_col_reg<24, true>,
-      (const void *)rd::k_rowi_reg<8>,         (const void *)rd::k_rowi_reg<24>,
-      (const void *)rd::k_rowi_vjp<8>,         (const void *)rd::k_rowi_vjp<24>,
+      (const void *)rd::k_rowf_reg<8, false>,     (const void *)rd::k_rowf_reg<8, true>,
+      (const void *)rd::k_rowf_reg<8, false, -1>, (const void *)rd::k_rowf_reg<8, true, -1>,
+      (const void *)rd::k_col_reg<8, false>,      (const void *)rd::k_col_reg<8, true>,
+      (const void *)rd::k_col_reg<16, false>,     (const void *)rd::k_col_reg<16, true>,
+      (const void *)rd::k_col_reg<8, false, -1>,  (const void *)rd::k_col_reg<8, true, -1>,
+      (const void *)rd::k_rowi_reg<8>,            (const void *)rd::k_rowi_reg<8, -1>,
+      (const void *)rd::k_rowi_vjp<8>,            (const void *)rd::k_rowi_vjp<8, -1>,
       (const void *)rd::k_rowf_reg<8, false, kRowLs[0]>,
       (const void *)rd::k_rowf_reg<8, true, kRowLs[0]>,
       (const void *)rd::k_rowf_reg<8, false, kRowLs[1]>,

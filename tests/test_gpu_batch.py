@@ -47,6 +47,44 @@ def test_batch_steps_bitwise_vs_oracle(hlib, olib):
             assert np.array_equal(f, mo.rhs(mo.state())), i
 
 
+def test_batch_rhs_timing_reports_classes_and_leaves_results_unchanged(hlib, olib):
+    """rdcu_batch_time: one entry per shape class (by dimension, then grid) with its
+    instance count, positive times and the algorithmic bytes of the stated
+    formula; timing between attempts leaves the steps equal to the oracle's."""
+    probs = mixed_set()
+    ys = [glibc_rand_ic(2, p.N, seed=1000 + i, A=1.0) for i, p in enumerate(probs)]
+    n = len(probs)
+    print("case batch timing", flush=True)
+    with Batch(hlib, probs, ys) as g, Batch(olib, probs, ys) as o:
+        tg, hg = np.zeros(n), np.full(n, 0.01)
+        to, ho = tg.copy(), hg.copy()
+        tg, hg, ag = g.step(tg, hg)
+        to, ho, ao = o.step(to, ho)
+        ms, classes = g.time_rhs(reps=5)
+        assert ms > 0
+        grids = [c["grid"] for c in classes]
+        assert grids == [[256, 1], [64, 64], [96, 96], [128, 64]]
+        assert [c["instances"] for c in classes] == [1, 2, 2, 1]
+        for c in classes:
+            assert c["ms"]["row_fwd"] > 0 and c["ms"]["row_inv"] > 0
+            assert (c["ms"]["col_mult"] > 0) == (c["ndim"] == 2)
+        # 2 fields; the CGLE and FitzHugh-Nagumo dictionaries reference the
+        # second derivatives of both fields (4 factors in 2-D, 2 in 1-D)
+        N = 64 * 64
+        assert classes[1]["bytes"] == {"row_fwd": 2 * 16 * 2 * N, "col_mult": 2 * (16 * N + 32 * N),
+                                       "row_inv": 2 * (32 * N + 32 * N)}
+        assert classes[0]["bytes"] == {"row_fwd": 16 * 2 * 256, "col_mult": 0.0,
+                                       "row_inv": 24 * 2 * 256}
+        ms_o, cls_o = o.time_rhs(reps=1)
+        assert ms_o > 0 and cls_o == []
+        for k in range(6):
+            tg, hg, ag = g.step(tg, hg)
+            to, ho, ao = o.step(to, ho)
+            assert np.array_equal(tg, to) and np.array_equal(hg, ho) and np.array_equal(ag, ao), k
+        for i in range(n):
+            assert np.array_equal(g.member(i).state(), o.member(i).state()), i
+
+
 def test_batch_equals_single_gpu_runs(hlib):
     """The sweep manifest's first instances: batched run == each alone."""
     insts = benchmark_manifest(512, seed=0)[::40]  # 13 instances, both kinds, mixed grids

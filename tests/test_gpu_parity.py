@@ -41,7 +41,7 @@ RHS_CASES = [
     ("2d-512", [512, 512], [200.0, 200.0], "cgle"),
     ("3d-16x24x32", [16, 24, 32], [10.0, 20.0, 30.0], "mixed"),
     ("3d-32", [32, 32, 32], [30.0, 30.0, 30.0], "cgle"),
-    ("2d-64x192", [64, 192], [40.0, 120.0], "cgle"),      # x length 3*2^k: 24 values/thread
+    ("2d-64x192", [64, 192], [40.0, 120.0], "cgle"),      # x length 3*2^k: radix-3 pass in LDS
     ("2d-96x384", [96, 384], [60.0, 240.0], "fhn"),
     ("2d-40x20-lds", [40, 20], [30.0, 15.0], "mixed"),    # radix-5 lengths: LDS kernels
     ("3d-10x12x8-lds", [10, 12, 8], [5.0, 6.0, 4.0], "cgle"),

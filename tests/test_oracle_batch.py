@@ -61,3 +61,14 @@ def test_manifest_is_deterministic_and_balanced():
             assert x.prob.terms[-1].coef == -x.params["c"]
         else:
             assert 2 <= x.params["delta"] <= 10 and 0.01 <= x.params["eps"] <= 0.1
+
+
+def test_batch_time_on_the_oracle(olib):
+    """rdcu_batch_time on the oracle: the wall clock of every instance's RHS,
+    no per-class entries."""
+    probs = [Problem(2, [32, 32], [20.0, 20.0], cgle_terms(2)),
+             Problem(2, [64], [30.0], cgle_terms(1))]
+    ys = [glibc_rand_ic(2, p.N, seed=3 + i, A=1.0) for i, p in enumerate(probs)]
+    with Batch(olib, probs, ys) as b:
+        ms, classes = b.time_rhs(reps=2)
+    assert ms > 0 and classes == []
