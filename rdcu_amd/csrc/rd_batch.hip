@@ -85,6 +85,10 @@ size_t rowi_lds(const BatchClass &cl, int G, int R) {
 // 256 points); other lengths run the run-time-plan kernels.
 constexpr int kRowLsB[] = {32, 64, 128};
 constexpr int kColLsB[] = {64, 128, 256};
+// and the 3*2^k lengths of the 96^2 and 192^2 grids (x lines 48 and 96,
+// strided lines 96 and 192)
+constexpr int kRowLsB3[] = {48, 96};
+constexpr int kColLsB3[] = {96, 192};
 
 template <int E, int LS>
 void go_rowf_b(rdcu_batch *b, const BatchClass &cl, dim3 g, dim3 bl, size_t lds, int per,
@@ -116,6 +120,8 @@ void launch_rowf_b(rdcu_batch *b, const BatchClass &cl, int src_kind, rd::StageC
     if (c->L == kRowLsB[0]) return go_rowf_b<E, kRowLsB[0]>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
     if (c->L == kRowLsB[1]) return go_rowf_b<E, kRowLsB[1]>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
     if (c->L == kRowLsB[2]) return go_rowf_b<E, kRowLsB[2]>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
+    if (c->L == kRowLsB3[0]) return go_rowf_b<E, kRowLsB3[0]>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
+    if (c->L == kRowLsB3[1]) return go_rowf_b<E, kRowLsB3[1]>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
     if (odd_len(c->L)) return go_rowf_b<E, -1>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
   }
   go_rowf_b<E, 0>(b, cl, gd, bl, lds, per, src_kind, sc, rpb, check);
@@ -145,6 +151,8 @@ void launch_col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, int chec
     if (cg.L == kColLsB[0]) return go_col_b<E, kColLsB[0]>(b, cl, gd, bl, lds, per, cg, fpw, check);
     if (cg.L == kColLsB[1]) return go_col_b<E, kColLsB[1]>(b, cl, gd, bl, lds, per, cg, fpw, check);
     if (cg.L == kColLsB[2]) return go_col_b<E, kColLsB[2]>(b, cl, gd, bl, lds, per, cg, fpw, check);
+    if (cg.L == kColLsB3[0]) return go_col_b<E, kColLsB3[0]>(b, cl, gd, bl, lds, per, cg, fpw, check);
+    if (cg.L == kColLsB3[1]) return go_col_b<E, kColLsB3[1]>(b, cl, gd, bl, lds, per, cg, fpw, check);
     if (odd_len(cg.L)) return go_col_b<E, -1>(b, cl, gd, bl, lds, per, cg, fpw, check);
   }
   go_col_b<E, 0>(b, cl, gd, bl, lds, per, cg, fpw, check);
@@ -598,7 +606,9 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
             (const void *)rd::k_rowi_b<8>, (const void *)rd::k_rowi_b<8, -1>,
             (const void *)rd::k_rowf_b<8, kRowLsB[0]>, (const void *)rd::k_rowf_b<8, kRowLsB[1]>,
             (const void *)rd::k_rowf_b<8, kRowLsB[2]>, (const void *)rd::k_col_b<8, kColLsB[0]>,
-            (const void *)rd::k_col_b<8, kColLsB[1]>, (const void *)rd::k_col_b<8, kColLsB[2]>})
+            (const void *)rd::k_col_b<8, kColLsB[1]>, (const void *)rd::k_col_b<8, kColLsB[2]>,
+            (const void *)rd::k_rowf_b<8, kRowLsB3[0]>, (const void *)rd::k_rowf_b<8, kRowLsB3[1]>,
+            (const void *)rd::k_col_b<8, kColLsB3[0]>, (const void *)rd::k_col_b<8, kColLsB3[1]>})
         HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
       if ((st = allow_big_lds())) return st;
       lds_raised = true;

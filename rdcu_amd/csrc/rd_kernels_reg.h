@@ -394,7 +394,7 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
 // lines (their radix-3 pass through LDS), need more than 256 registers, so
 // their blocks stay at 256 threads and nothing spills.
 template <int E, int LS = 0>
-constexpr int col_max_threads() { return E == 8 && LS >= 0 ? 512 : 256; }
+constexpr int col_max_threads() { return E == 8 && (LS == 0 || (LS > 0 && LS % 3 != 0)) ? 512 : 256; }
 
 template <int E, bool NT, int LS = 0, bool TG = false>
 __global__ __launch_bounds__((col_max_threads<E, LS>())) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,

@@ -386,6 +386,9 @@ int reg_elems(int L) {
 // and 512 points.
 constexpr int kRowLs[] = {128, 256};
 constexpr int kColLs[] = {256, 512};
+// with 8 values per thread also the 3*2^k lengths of 96- and 192-point grids
+constexpr int kRowLs3[] = {48, 96};
+constexpr int kColLs3[] = {96, 192};
 
 template <int E, bool NT, int LS>
 void go_rowf(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, int check, int src_kind,
@@ -404,6 +407,8 @@ void go_rowf_l(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, int check, int src_kind,
   if constexpr (E == 8) {
     if (c->L == kRowLs[0]) return go_rowf<E, NT, kRowLs[0]>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
     if (c->L == kRowLs[1]) return go_rowf<E, NT, kRowLs[1]>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
+    if (c->L == kRowLs3[0]) return go_rowf<E, NT, kRowLs3[0]>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
+    if (c->L == kRowLs3[1]) return go_rowf<E, NT, kRowLs3[1]>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
   }
   if (odd_len(c->L)) return go_rowf<E, NT, -1>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
   go_rowf<E, NT, 0>(c, g, b, lds, check, src_kind, src_ptr, sc, rpb);
@@ -457,6 +462,8 @@ void go_col_l(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, const rd::FftPlan &p, con
                          c->st, check, c->bufs, fl, mode, fpw);
       return;
     }
+    if (cg.L == kColLs3[0]) return go_col<E, NT, kColLs3[0]>(c, g, b, lds, p, cg, check, fl, mode, fpw);
+    if (cg.L == kColLs3[1]) return go_col<E, NT, kColLs3[1]>(c, g, b, lds, p, cg, check, fl, mode, fpw);
     if (odd_len(cg.L)) return go_col<E, NT, -1>(c, g, b, lds, p, cg, check, fl, mode, fpw);
   }
   go_col<E, NT, 0>(c, g, b, lds, p, cg, check, fl, mode, fpw);
@@ -1077,6 +1084,14 @@ int allow_big_lds() {
       (const void *)rd::k_col_reg<16, true, kColLs[0]>,
       (const void *)rd::k_col_reg<16, false, kColLs[1]>,
       (const void *)rd::k_col_reg<16, true, kColLs[1]>,
+      (const void *)rd::k_rowf_reg<8, false, kRowLs3[0]>,
+      (const void *)rd::k_rowf_reg<8, true, kRowLs3[0]>,
+      (const void *)rd::k_rowf_reg<8, false, kRowLs3[1]>,
+      (const void *)rd::k_rowf_reg<8, true, kRowLs3[1]>,
+      (const void *)rd::k_col_reg<8, false, kColLs3[0]>,
+      (const void *)rd::k_col_reg<8, true, kColLs3[0]>,
+      (const void *)rd::k_col_reg<8, false, kColLs3[1]>,
+      (const void *)rd::k_col_reg<8, true, kColLs3[1]>,
       (const void *)rd::k_col_reg<8, false, 0, true>,
       (const void *)rd::k_col_reg<8, true, 0, true>};
   for (const void *f : fs)
