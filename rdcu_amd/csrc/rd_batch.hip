@@ -34,6 +34,7 @@ struct BatchClass {
 struct rdcu_batch {
   int dev = 0;
   hipStream_t stream = nullptr;
+  hipStream_t ls = nullptr;  // stream the launch helpers enqueue on (stream, or a class's)
   std::vector<std::unique_ptr<rdcu_ctx>> members;
   std::vector<int> pos;  // member -> device state slot (= position in the full table)
   std::vector<int> tab;  // current table position -> member
@@ -43,8 +44,15 @@ struct rdcu_batch {
   rd::DevState *d_st = nullptr, *h_st = nullptr;  // instance table order
   int *d_done = nullptr, *h_done = nullptr;
   double *d_t = nullptr, *d_h = nullptr, *d_t1 = nullptr, *h_buf = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  // One attempt of each class's instances is one graph, replayed on the
+  // class's own stream, so that the classes' launches overlap instead of
+  // adding up; fork and join events order them after and before the work on
+  // `stream` (state set-up, done flags).
+  std::vector<hipStream_t> cstream;
+  std::vector<hipGraphExec_t> cexec;  // null for a class with no instance in the table
+  std::vector<hipEvent_t> cjoin;
+  hipEvent_t fork = nullptr;
+  bool captured = false;
   hipModule_t jit_mod = nullptr;
   std::string jit_log = "off";
   bool k1_valid = false;
@@ -52,9 +60,13 @@ struct rdcu_batch {
   long long nsync = 0;
 
   ~rdcu_batch() {
+    for (hipStream_t s : cstream) (void)hipStreamSynchronize(s);
     if (stream) (void)hipStreamSynchronize(stream);
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
+    for (hipGraphExec_t e : cexec)
+      if (e) (void)hipGraphExecDestroy(e);
+    for (hipEvent_t e : cjoin) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    for (hipStream_t s : cstream) (void)hipStreamDestroy(s);
     if (jit_mod) (void)hipModuleUnload(jit_mod);
     members.clear();  // before the stream and state they borrow
     for (void *p : {(void *)d_inst, (void *)d_st, (void *)d_done, (void *)d_t, (void *)d_h,
@@ -94,14 +106,14 @@ template <int E, int LS>
 void go_rowf_b(rdcu_batch *b, const BatchClass &cl, dim3 g, dim3 bl, size_t lds, int per,
                int src_kind, const rd::StageCoef &sc, int rpb, int check) {
   const rdcu_ctx *c = cl.c0;
-  hipLaunchKernelGGL((rd::k_rowf_b<E, LS>), g, bl, lds, b->stream, b->d_inst + cl.off, cl.cnt, per,
+  hipLaunchKernelGGL((rd::k_rowf_b<E, LS>), g, bl, lds, b->ls, b->d_inst + cl.off, cl.cnt, per,
                      c->px, c->twN, src_kind, sc, rpb, check);
 }
 
 template <int E, int LS>
 void go_col_b(rdcu_batch *b, const BatchClass &cl, dim3 g, dim3 bl, size_t lds, int per,
               const rd::ColGeom &cg, int fpw, int check) {
-  hipLaunchKernelGGL((rd::k_col_b<E, LS>), g, bl, lds, b->stream, b->d_inst + cl.off, cl.cnt, per,
+  hipLaunchKernelGGL((rd::k_col_b<E, LS>), g, bl, lds, b->ls, b->d_inst + cl.off, cl.cnt, per,
                      cl.c0->p0, cg, 1, fpw, check);
 }
 
@@ -179,14 +191,14 @@ void launch_rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_ki
       int a_src = src_kind, a_dk = dst_kind, a_ds = dst_slot, a_err = err, a_G = G, a_chk = check;
       void *args[] = {&in, &cnt, &rows, &px, &tw, &a_src, &a_dk, &a_ds, &a_err, &ec, &nx, &a_G,
                       &a_chk, &region};
-      (void)hipModuleLaunchKernel(gr.fn, grid, 1, 1, block, 1, 1, (unsigned)lds, b->stream, args,
+      (void)hipModuleLaunchKernel(gr.fn, grid, 1, 1, block, 1, 1, (unsigned)lds, b->ls, args,
                                   nullptr);
     } else if (odd_len(c->L)) {
-      hipLaunchKernelGGL((rd::k_rowi_b<E, -1>), dim3(grid), dim3(block), lds, b->stream, in, cnt,
+      hipLaunchKernelGGL((rd::k_rowi_b<E, -1>), dim3(grid), dim3(block), lds, b->ls, in, cnt,
                          rows, c->px, c->twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check,
                          region);
     } else {
-      hipLaunchKernelGGL(rd::k_rowi_b<E>, dim3(grid), dim3(block), lds, b->stream, in, cnt, rows,
+      hipLaunchKernelGGL(rd::k_rowi_b<E>, dim3(grid), dim3(block), lds, b->ls, in, cnt, rows,
                          c->px, c->twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check,
                          region);
     }
@@ -215,21 +227,38 @@ void rhs_b(rdcu_batch *b, const BatchClass &cl, int dst_kind, int slot, int chec
   rowi_b(b, cl, rd::SRC_Y, dst_kind, slot, 0, none, cl.G, cl.R, check);
 }
 
-// One DP45 attempt of every active instance (the single-instance sequence of
-// attempt_launches_reg per class), then step control for all.
-int enqueue_attempt_b(rdcu_batch *b) {
-  for (const BatchClass &cl : b->classes) {
-    rowf_b(b, cl, rd::SRC_STAGE, stage_coef(2), cl.rpb, cl.grid_rf, 1);
-    for (int s = 2; s <= 7; ++s) {
-      col_b(b, cl, cl.tc, cl.fpw, 1);
-      rd::StageCoef nx = s < 7 ? stage_coef(s + 1) : rd::StageCoef{};
-      const int dk = s < 7 ? rd::DST_KS : rd::DST_K7;
-      rowi_b(b, cl, rd::SRC_YT, dk, s - 1, s == 7 && !b->fixed, nx, cl.G, cl.R, 1);
-    }
+// One DP45 attempt of every active instance of a class (the single-instance
+// sequence of attempt_launches_reg), then their step control, on b->ls.
+int enqueue_attempt_b(rdcu_batch *b, const BatchClass &cl) {
+  rowf_b(b, cl, rd::SRC_STAGE, stage_coef(2), cl.rpb, cl.grid_rf, 1);
+  for (int s = 2; s <= 7; ++s) {
+    col_b(b, cl, cl.tc, cl.fpw, 1);
+    rd::StageCoef nx = s < 7 ? stage_coef(s + 1) : rd::StageCoef{};
+    const int dk = s < 7 ? rd::DST_KS : rd::DST_K7;
+    rowi_b(b, cl, rd::SRC_YT, dk, s - 1, s == 7 && !b->fixed, nx, cl.G, cl.R, 1);
   }
-  hipLaunchKernelGGL(rd::k_control_b, dim3((unsigned)b->tab.size()), dim3(256), 0, b->stream,
-                     b->d_inst, b->d_done);
+  hipLaunchKernelGGL(rd::k_control_b, dim3((unsigned)cl.cnt), dim3(256), 0, b->ls,
+                     b->d_inst + cl.off, b->d_done + cl.off);
   HIPCHECK(hipGetLastError());
+  return RDCU_OK;
+}
+
+// k attempts of every instance in the table: each class's graph k times on
+// its own stream, after the work already on b->stream and before what
+// follows there.
+int replay_b(rdcu_batch *b, long long k) {
+  HIPCHECK(hipEventRecord(b->fork, b->stream));
+  const int nc = (int)b->classes.size();
+  for (int c = 0; c < nc; ++c)
+    if (b->cexec[c]) HIPCHECK(hipStreamWaitEvent(b->cstream[c], b->fork, 0));
+  for (long long r = 0; r < k; ++r)
+    for (int c = 0; c < nc; ++c)
+      if (b->cexec[c]) HIPCHECK(hipGraphLaunch(b->cexec[c], b->cstream[c]));
+  for (int c = 0; c < nc; ++c)
+    if (b->cexec[c]) {
+      HIPCHECK(hipEventRecord(b->cjoin[c], b->cstream[c]));
+      HIPCHECK(hipStreamWaitEvent(b->stream, b->cjoin[c], 0));
+    }
   return RDCU_OK;
 }
 
@@ -416,25 +445,37 @@ int build_table(rdcu_batch *b, const std::vector<char> &keep) {
   return RDCU_OK;
 }
 
-// Captures one attempt of the instances in the current table.
+// Captures one attempt of each class's instances in the current table (the
+// caller has synchronised the class streams: their previous graphs are idle).
 int capture_b(rdcu_batch *b) {
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->gexec = nullptr;
-  b->graph = nullptr;
-  HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  int st = enqueue_attempt_b(b);
-  hipGraph_t graph = nullptr;
-  hipError_t ce = hipStreamEndCapture(b->stream, &graph);
-  if (st) return st;
-  if (ce != hipSuccess) return fail(RDCU_EDEVICE, std::string("capture: ") + hipGetErrorString(ce));
-  b->graph = graph;
-  HIPCHECK(hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0));
+  b->captured = false;
+  for (size_t c = 0; c < b->classes.size(); ++c) {
+    if (b->cexec[c]) (void)hipGraphExecDestroy(b->cexec[c]);
+    b->cexec[c] = nullptr;
+    const BatchClass &cl = b->classes[c];
+    if (cl.cnt == 0) continue;
+    HIPCHECK(hipStreamBeginCapture(b->cstream[c], hipStreamCaptureModeThreadLocal));
+    b->ls = b->cstream[c];
+    int st = enqueue_attempt_b(b, cl);
+    b->ls = b->stream;
+    hipGraph_t graph = nullptr;
+    hipError_t ce = hipStreamEndCapture(b->cstream[c], &graph);
+    if (st) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return st;
+    }
+    if (ce != hipSuccess) return fail(RDCU_EDEVICE, std::string("capture: ") + hipGetErrorString(ce));
+    const hipError_t ie = hipGraphInstantiate(&b->cexec[c], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess)
+      return fail(RDCU_EDEVICE, std::string("instantiate: ") + hipGetErrorString(ie));
+  }
+  b->captured = true;
   return RDCU_OK;
 }
 
 int full_table(rdcu_batch *b) {
-  if (b->table_full && b->gexec) return RDCU_OK;
+  if (b->table_full && b->captured) return RDCU_OK;
   int st = build_table(b, std::vector<char>(b->members.size(), 1));
   if (st) return st;
   return capture_b(b);
@@ -498,6 +539,8 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
   b->fixed = fixedstep ? 1 : 0;
   HIPCHECK(hipSetDevice(device));
   HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  b->ls = b->stream;
+  HIPCHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   {
     void *q = nullptr;
     HIPCHECK(hipMalloc(&q, sizeof(rd::DevState) * count));
@@ -653,6 +696,15 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
               b->jit_log.c_str(), cl.tune_log.c_str(), cl.rpb, cl.tc, cl.fpw, cl.G, cl.R,
               cl.grid_rf);
   }
+  for (size_t c = 0; c < b->classes.size(); ++c) {  // one stream per class (default priority)
+    hipStream_t cs = nullptr;
+    hipEvent_t ce = nullptr;
+    HIPCHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    b->cstream.push_back(cs);
+    HIPCHECK(hipEventCreateWithFlags(&ce, hipEventDisableTiming));
+    b->cjoin.push_back(ce);
+    b->cexec.push_back(nullptr);
+  }
   if ((st = capture_b(b.get()))) return st;
   HIPCHECK(hipStreamSynchronize(b->stream));
   *out = b.release();
@@ -748,7 +800,7 @@ int rdcu_batch_step(rdcu_batch *b, double *t, double *h, double hmin, double hma
   if ((st = full_table(b))) return st;
   if ((st = begin_b(b, t, h, nullptr, -INFINITY, hmax, 0))) return st;
   if ((st = ensure_k1_b(b, false))) return st;
-  HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
+  if ((st = replay_b(b, 1))) return st;
   if ((st = read_states_b(b))) return st;
   for (int i = 0; i < ninst(b); ++i) {
     const rd::DevState &s = b->h_st[b->pos[i]];
@@ -776,10 +828,18 @@ int rdcu_batch_run(rdcu_batch *b, double *t, double *h, double hmin, double hmax
   if ((st = ensure_k1_b(b, true))) return st;
   long long done = 0;
   bool all = false;
+  // with RDCU_VERBOSE, one line per table: the graph replays it ran, its
+  // instances, and the wall time it took (where a sweep's time goes)
+  const char *vb = getenv("RDCU_VERBOSE");
+  const bool verbose = vb && *vb && *vb != '0';
+  auto now = []() { return std::chrono::steady_clock::now(); };
+  auto t_tab = now();
+  long long tab_replays = 0;
   while (!all && done < max_attempts) {
     const long long k = std::min<long long>(kChunkMax, max_attempts - done);
-    for (long long r = 0; r < k; ++r) HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
+    if ((st = replay_b(b, k))) return st;
     done += k;
+    tab_replays += k;
     const int tn = (int)b->tab.size();
     HIPCHECK(hipMemcpyAsync(b->h_done, b->d_done, sizeof(int) * tn, hipMemcpyDeviceToHost,
                             b->stream));
@@ -793,6 +853,13 @@ int rdcu_batch_run(rdcu_batch *b, double *t, double *h, double hmin, double hmax
         keep[b->tab[p]] = 1;
       }
     all = active == 0;
+    if (verbose && (all || done >= max_attempts || 2 * active <= tn)) {
+      const double sec = std::chrono::duration<double>(now() - t_tab).count();
+      fprintf(stderr, "rdcu batch table: %d instances, %lld replays, %.3f s (%.1f us per replay)\n",
+              tn, tab_replays, sec, 1e6 * sec / (double)tab_replays);
+      t_tab = now();
+      tab_replays = 0;
+    }
     // once half the table has finished, later attempts launch only the
     // instances still integrating (a new table and graph)
     if (!all && done < max_attempts && 2 * active <= tn) {

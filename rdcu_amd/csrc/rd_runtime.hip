@@ -15,6 +15,7 @@
 #include <hip/hiprtc.h>
 
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
