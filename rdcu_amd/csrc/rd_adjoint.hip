@@ -420,6 +420,21 @@ int loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const doub
   rd::DenseCoef dc;
   for (int j = 0; j < 7; ++j)
     for (int q = 0; q < 4; ++q) dc.p[j][q] = PP[j][q];
+  // with RDCU_VERBOSE: HIP-event times of the forward pass and the reverse
+  // sweep, and the kernels of one reverse step (stderr)
+  const char *vb = getenv("RDCU_VERBOSE");
+  struct PhaseEvents {
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+    ~PhaseEvents() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } pev;
+  const bool timing = vb && *vb && *vb != '0';
+  if (timing)
+    for (hipEvent_t &x : pev.e) HIPCHECK(hipEventCreate(&x));
+  size_t step_nodes = 0;
+  if (timing) HIPCHECK(hipEventRecord(pev.e[0], c->stream));
   int o = 0;
   for (long long k = 0; k < ns; ++k) {
     if (grad)
@@ -438,6 +453,7 @@ int loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const doub
     }
   }
   HIPCHECK(hipGetLastError());
+  if (timing) HIPCHECK(hipEventRecord(pev.e[1], c->stream));
   c->nrhs += 6 * ns;
   hipLaunchKernelGGL(rd::k_set_sched, dim3(1), dim3(1), 0, c->stream, c->st, c->fixed, nullptr,
                      0LL, 0);
@@ -630,16 +646,19 @@ int loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const doub
       if (g) (void)hipGraphDestroy(g);
       return s2 ? s2 : fail(RDCU_EDEVICE, std::string("reverse step capture: ") + hipGetErrorString(ce));
     }
+    if (par == 0 && timing) (void)hipGraphGetNodes(g, nullptr, &step_nodes);
     const hipError_t ie = hipGraphInstantiate(&ex.x[par], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ie != hipSuccess)
       return fail(RDCU_EDEVICE, std::string("reverse step graph: ") + hipGetErrorString(ie));
   }
   hipLaunchKernelGGL(rd::k_adj_setk, dim3(1), dim3(64), 0, c->stream, kdev, ns - 1);
+  if (timing) HIPCHECK(hipEventRecord(pev.e[2], c->stream));
   long long i = 0;
   if (ex.x[2])
     for (; i + kGraphAttempts <= ns; i += kGraphAttempts) HIPCHECK(hipGraphLaunch(ex.x[2], c->stream));
   for (; i < ns; ++i) HIPCHECK(hipGraphLaunch(ex.x[i & 1], c->stream));
+  if (timing) HIPCHECK(hipEventRecord(pev.e[3], c->stream));
   if (!kst) c->nrhs += 7 * ns;
   lam = buf[ns & 1];
   std::vector<double> hp((size_t)std::max(1, nt) * c->rows);
@@ -649,6 +668,15 @@ int loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const doub
     HIPCHECK(hipMemcpyAsync(grad_y0, lam, nN * sizeof(double),
                             dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
+  if (timing) {
+    float fwd = 0, rev = 0;
+    HIPCHECK(hipEventElapsedTime(&fwd, pev.e[0], pev.e[1]));
+    HIPCHECK(hipEventElapsedTime(&rev, pev.e[2], pev.e[3]));
+    fprintf(stderr,
+            "rdcu loss_grad: %lld steps; forward with checkpoints %.3f ms (%.1f us per step); "
+            "reverse sweep %.3f ms (%.1f us per step, %zu graph nodes per step)\n",
+            ns, fwd, 1e3 * fwd / (double)ns, rev, 1e3 * rev / (double)ns, step_nodes);
+  }
   if (grad_coef)
     for (int t = 0; t < nt; ++t) {
       double sacc = 0.0;
