@@ -323,8 +323,8 @@ int autotune_b(rdcu_batch *b, BatchClass &cl) {
   }
   best = 1e30;
   // factor lines per group, also wider than the factor count (powers of two;
-  // idle lines, but more threads per row: with 24 values per thread a block
-  // holds one row, and at short rows G*T is a fraction of a wave)
+  // idle lines, but more threads per row: at short rows G*T is a fraction of
+  // a wave)
   const int Gf = std::max(c->n, cl.nf_max);
   for (int G = c->n; G * T <= 256; G = G < Gf ? G + 1 : (G == Gf ? pow2_above(Gf) : 2 * G)) {
     for (int R = 1; R * G * T <= 256 && c->rows % R == 0 && (R == 1 || cl.Ex <= 8); R *= 2) {

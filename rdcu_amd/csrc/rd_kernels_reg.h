@@ -664,8 +664,8 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
     ks[0] = b.k[st->k1s];
     for (int j = 1; j < 6; ++j) ks[j] = b.k[j];
     const int half = Nx / 2;
-    // items in flight per thread: 4 (24 values per thread have no registers
-    // left for more than one)
+    // items in flight per thread: 4 (kernels with more than 8 values per
+    // thread have no registers left for more than one)
     constexpr int AXU = E <= 8 ? 4 : 1;
 #pragma unroll AXU
     for (int w = tid; w < n * half; w += nt) {
