@@ -60,9 +60,13 @@ def measured_traffic(kernel):
     sym = KERNEL_SYMBOL.get(kernel)
     if not sym or not os.path.exists(PROFILE):
         return None
+    in_pmc = False  # rows of the counter table only (after its header)
     for line in open(PROFILE):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 6 and sym in cells[0]:
+        if len(cells) == 6 and cells[-1] == "scaled total MB":
+            in_pmc = True
+            continue
+        if in_pmc and len(cells) == 6 and sym in cells[0]:
             try:
                 return float(cells[5]) * 1e6
             except ValueError:
