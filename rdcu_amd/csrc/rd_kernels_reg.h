@@ -483,7 +483,10 @@ __host__ __device__ inline long rowi_region(int L, int Nx, int n, int nf, int nw
 // One grid row r, by the nt threads tid = 0..nt-1 of a row group whose LDS
 // region starts at smem.  Every row group of a block runs the same barrier
 // sequence.
-template <int E, class DE, int LS = 0, class GK = RunGeom, bool TWR = false>
+// LEAN: fewer points and stage-sum items per thread in flight (half the
+// registers of those phases), for kernels compiled to keep two wavefronts per
+// SIMD (where a CU holds many rows, 3-D).
+template <int E, class DE, int LS = 0, class GK = RunGeom, bool TWR = false, bool LEAN = false>
 __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
                                           const FacList &fl, const DictP &dict, int src_kind,
@@ -594,7 +597,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
     atol = st->atol;
     rtol = st->rtol;
   }
-  constexpr int XM = E <= 8 ? 4 : 2;  // points per thread per pass (register budget)
+  constexpr int XM = E <= 8 && !LEAN ? 4 : 2;  // points per thread per pass (register budget)
   for (int xb = 0; xb < Nx; xb += XM * nt) {
     const int x0 = xb + tid;
     for (int f = 0; f < n; ++f) {
@@ -666,7 +669,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
     const int half = Nx / 2;
     // items in flight per thread: 4 (kernels with more than 8 values per
     // thread have no registers left for more than one)
-    constexpr int AXU = E <= 8 ? 4 : 1;
+    constexpr int AXU = E <= 8 && !LEAN ? 4 : 1;
 #pragma unroll AXU
     for (int w = tid; w < n * half; w += nt) {
       const int f = w / half, j2 = 2 * (w - f * half);
@@ -738,7 +741,7 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
 // Block = R rows x G*T threads (R = blockDim / (G*T)), row r = blockIdx*R +
 // sub.  With more than 8 values per thread the per-row thread range and LDS
 // offset would cost registers such kernels do not have, so they take R = 1.
-template <int E, class DE, int LS = 0, class GK = RunGeom, bool TWR = false>
+template <int E, class DE, int LS = 0, class GK = RunGeom, bool TWR = false, bool LEAN = false>
 __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
                                           const cx *__restrict__ twN, DevState *st, const Bufs &b,
                                           const FacList &fl, const DictP &dict, int src_kind,
@@ -753,11 +756,11 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
     const int nf = GK::nf >= 0 ? GK::nf : fl.nf;
     const long region = rowi_region(L, Nx, n, nf, dict.nwords, G);
     const int gt = G * (L / E), sub = threadIdx.x / gt;
-    rowi_body<E, DE, LS, GK, TWR>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl,
+    rowi_body<E, DE, LS, GK, TWR, LEAN>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl,
                              dict, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,
                              smem + sub * region, threadIdx.x - sub * gt, gt, chk);
   } else {
-    rowi_body<E, DE, LS, GK, TWR>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
+    rowi_body<E, DE, LS, GK, TWR, LEAN>(blockIdx.x, g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind,
                              dst_slot, dst_ptr, err, ec, nx, G, smem, threadIdx.x, blockDim.x,
                              chk);
   }

@@ -166,7 +166,9 @@ struct rdcu_ctx {
   hipModule_t jit_mod = nullptr;
   hipFunction_t jit_rowi = nullptr;
   hipFunction_t jit_rowi_tw = nullptr;  // the same with the twiddles in registers
-  int ri_tw = 0;                        // the tuner's choice between the two
+  // both compiled lean (fewer values in flight) for two wavefronts per SIMD
+  hipFunction_t jit_rowi_lean = nullptr, jit_rowi_lean_tw = nullptr;
+  int ri_tw = 0;  // the tuner's choice: 0 plain, 1 tw, 2 lean, 3 lean tw
   // 1-D: a whole attempt in one workgroup (null if not built or not eligible)
   hipFunction_t jit_att = nullptr;
   size_t lds_att = 0;
@@ -491,6 +493,18 @@ size_t rowi_lds(const rdcu_ctx *c, int G, int R) {
          sizeof(cx);
 }
 
+// The generated x-inverse kernel of variant v (0 plain, 1 twiddles in
+// registers, 2 lean, 3 lean with the twiddles in registers); null if that
+// variant was not built.
+hipFunction_t jit_rowi_variant(const rdcu_ctx *c, int v) {
+  switch (v) {
+    case 1: return c->jit_rowi_tw;
+    case 2: return c->jit_rowi_lean;
+    case 3: return c->jit_rowi_lean_tw;
+    default: return c->jit_rowi;
+  }
+}
+
 template <int E>
 void launch_rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, int dst_kind,
                  int dst_slot, double *dst_ptr, int err, rd::StageCoef nx, int G, int R) {
@@ -518,7 +532,7 @@ void launch_rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, in
     int a_G = G;
     void *args[] = {&a_g, &a_px, &a_tw, &a_st, &a_check, &a_b, &a_fl, &a_d, &a_src,
                     &a_sp, &a_dk, &a_ds, &a_dp, &a_err, &a_ec, &a_nx, &a_G};
-    hipFunction_t fn = c->ri_tw && c->jit_rowi_tw ? c->jit_rowi_tw : c->jit_rowi;
+    hipFunction_t fn = jit_rowi_variant(c, c->ri_tw);
     (void)hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, (unsigned)lds, c->stream, args,
                                 nullptr);
     return;
@@ -826,14 +840,23 @@ void build_jit(rdcu_ctx *c) {
   src += "  rd::rowi_rows<" + E + ", rd::JitDict, " + std::to_string(c->L) +
          ", rd::JitGeom>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, dst_slot, "
          "dst_ptr, err, ec, nx, G, check_active ? st : nullptr);\n}\n";
-  // the same kernel with the transforms' twiddles in registers (the tuner
-  // times both)
+  // the same kernel with the transforms' twiddles in registers, and both
+  // compiled lean for two wavefronts per SIMD: at 256^3 the full kernel needs
+  // more than 256 registers and a SIMD holds one of its wavefronts, four rows
+  // per CU where the LDS has room for seven (the tuner times all four)
   if (c->Ex == 8) {
-    std::string tw = sig;
-    tw.replace(tw.find("rdcu_rowi_jit"), 13, "rdcu_rowi_jit_tw");
-    src += tw + "  rd::rowi_rows<" + E + ", rd::JitDict, " + std::to_string(c->L) +
-           ", rd::JitGeom, true>(g, px, twN, st, b, fl, dict, src_kind, src_ptr, dst_kind, "
-           "dst_slot, dst_ptr, err, ec, nx, G, check_active ? st : nullptr);\n}\n";
+    const struct { const char *name; const char *bounds; const char *args; } vs[] = {
+        {"rdcu_rowi_jit_tw", "256", "true, false"},
+        {"rdcu_rowi_jit_lean", "256, 2", "false, true"},
+        {"rdcu_rowi_jit_lean_tw", "256, 2", "true, true"}};
+    for (const auto &v : vs) {
+      std::string sv = sig;
+      sv.replace(sv.find("__launch_bounds__(256)"), 22, std::string("__launch_bounds__(") + v.bounds + ")");
+      sv.replace(sv.find("rdcu_rowi_jit"), 13, v.name);
+      src += sv + "  rd::rowi_rows<" + E + ", rd::JitDict, " + std::to_string(c->L) +
+             ", rd::JitGeom, " + v.args + ">(g, px, twN, st, b, fl, dict, src_kind, src_ptr, "
+             "dst_kind, dst_slot, dst_ptr, err, ec, nx, G, check_active ? st : nullptr);\n}\n";
+    }
   }
   const size_t att_lds = attempt1d_lds(c);
   if (att_lds)
@@ -853,10 +876,14 @@ void build_jit(rdcu_ctx *c) {
     c->jit_rowi = nullptr;
     return;
   }
-  if (c->Ex == 8 &&
-      (hipModuleGetFunction(&c->jit_rowi_tw, c->jit_mod, "rdcu_rowi_jit_tw") != hipSuccess ||
-       jit_spills(c->jit_rowi_tw)))
-    c->jit_rowi_tw = nullptr;
+  // a variant that does not load or would spill is left out
+  auto variant = [&](hipFunction_t *f, const char *name) {
+    if (c->Ex != 8 || hipModuleGetFunction(f, c->jit_mod, name) != hipSuccess || jit_spills(*f))
+      *f = nullptr;
+  };
+  variant(&c->jit_rowi_tw, "rdcu_rowi_jit_tw");
+  variant(&c->jit_rowi_lean, "rdcu_rowi_jit_lean");
+  variant(&c->jit_rowi_lean_tw, "rdcu_rowi_jit_lean_tw");
   if (att_lds && hipModuleGetFunction(&c->jit_att, c->jit_mod, "rdcu_attempt1d_jit") == hipSuccess &&
       !jit_spills(c->jit_att))
     c->lds_att = att_lds;
@@ -930,7 +957,7 @@ int fixed_shapes(rdcu_ctx *c, const char *spec) {
   c->cr_fpw = fpw;
   c->ri_G = G;
   c->ri_R = R;
-  c->ri_tw = tw && c->jit_rowi_tw ? 1 : 0;
+  c->ri_tw = tw >= 0 && tw <= 3 && jit_rowi_variant(c, tw) ? tw : 0;
   c->tune_log = std::string("fixed ") + spec;
   return RDCU_OK;
 }
@@ -1033,7 +1060,8 @@ int autotune(rdcu_ctx *c) {
   // (with a generated kernel, also its variant with the twiddles in
   // registers, timed on the fused pass of an attempt's stage, where it runs)
   rd::StageCoef nx3 = stage_coef(3);
-  for (int tw = 0; tw <= (c->jit_rowi_tw ? 1 : 0); ++tw) {
+  for (int tw = 0; tw <= 3; ++tw) {
+    if (tw > 0 && !jit_rowi_variant(c, tw)) continue;
     c->ri_tw = tw;
     for (int G = c->n; G * T <= 256; G = G < Gf ? G + 1 : (G == Gf ? pow2_above(Gf) : 2 * G)) {
       for (int R = 1; R * G * T <= 256 && (R == 1 || c->Ex <= 8); R *= 2) {

@@ -309,17 +309,22 @@ def test_sixteen_values_per_thread_columns_bitwise(hlib, olib, monkeypatch, Ns, 
                                           ([512, 512], [200.0, 200.0], "2,3,2,4,1,8,128"),
                                           ([96, 48, 64], [40.0, 20.0, 30.0], "4,2,1,4,1,8,11"),
                                           ([512, 512], [200.0, 200.0], "2,4,1,4,1,8,0,1"),
-                                          ([32, 64, 256], [30.0, 40.0, 90.0], "4,4,2,4,1,8,0,1")],
+                                          ([32, 64, 256], [30.0, 40.0, 90.0], "4,4,2,4,1,8,0,1"),
+                                          ([512, 512], [200.0, 200.0], "2,4,1,4,2,8,0,2"),
+                                          ([32, 64, 256], [30.0, 40.0, 90.0], "4,4,2,4,1,8,0,2"),
+                                          ([32, 64, 256], [30.0, 40.0, 90.0], "4,4,2,4,1,8,0,3"),
+                                          ([96, 192], [40.0, 80.0], "4,4,1,4,1,8,0,3")],
                          ids=["2d-512-fpw2", "2d-512-fpw1", "2d-256-e16", "3d-256z",
                               "2d-512-64tiles", "2d-512-128tiles-fpw2", "3d-11tiles",
-                              "2d-512-twreg", "3d-256x-twreg"])
+                              "2d-512-twreg", "3d-256x-twreg", "2d-512-lean-R2", "3d-256x-lean",
+                              "3d-256x-lean-twreg", "2d-96x192-lean-twreg"])
 def test_compiled_length_kernels_bitwise(hlib, olib, monkeypatch, Ns, Ls, shapes):
     """Kernels compiled for one transform length (128/256-point x lines,
     256/512-point strided lines), with two factors per block (fpw = 2) or one
     (fpw = 1), 8 or 16 values per thread, column tiles of unequal widths
     (the 257 columns of a 512-point row over 64 or 128 tiles; 33 over 11),
-    and the generated x-inverse kernel with its twiddles in registers: the
-    oracle's bits."""
+    and the generated x-inverse kernel with its twiddles in registers, and
+    both compiled lean for two wavefronts per SIMD: the oracle's bits."""
     prob = make_problem(Ns, Ls, "cgle")
     y0 = glibc_rand_ic(prob.n, prob.N, seed=7, A=1.0)
     monkeypatch.setenv("RDCU_SHAPES", shapes)
