@@ -1876,12 +1876,16 @@ int rdcu_rhs_noalloc(rdcu_ctx *c, const double *y, double *f_out, double *Y_out)
                                  " bytes of device scratch, more than the " +
                                  std::to_string(kScratchBytes) + " the library holds per GPU");
   std::lock_guard<std::mutex> lk(scratch_mutex(c->dev));
-  // the context's own buffers are swapped out for the call, then restored
+  // the context's own buffers are swapped out for the call, then restored;
+  // on every return, an error one included, the stream is drained first, so
+  // that no work reading or writing the array is in flight once the lock is
+  // released (the guard is destroyed after this)
   struct Swap {
     rdcu_ctx *c;
     rd::Bufs b;
     double *in, *out, *Yr, *Yc;
     ~Swap() {
+      (void)hipStreamSynchronize(c->stream);
       c->bufs = b;
       c->rhs_in = in;
       c->rhs_out = out;
