@@ -456,10 +456,14 @@ __device__ __forceinline__ void c2r_pre(const Grid &g, const FacList &fl, const 
 
 // LDS of one row of the x-inverse pass (complex units).  The value rows
 // (n identity rows, then nf factor rows, Nx doubles each) come first.  The
-// FFT lines start where the rows of the last factor group begin: those rows
-// are written only after the group's transform, when the lines are free.
-// After them come the n stage-result (or error-square) rows, then the
-// dictionary program.  The twiddles are read from global memory (cached).
+// factors are transformed in groups of G lines, the first group holding the
+// remainder nf - (ng-1)*G, so that the last group is a full one.  The FFT
+// lines start where the rows of the last group begin: those rows are written
+// only after the group's transform, when the lines are free.  After them come
+// the n stage-result (or error-square) rows, then (for the interpreted
+// dictionary) the program.  The twiddles are read from global memory
+// (cached).  At 256^3 (L = 128, n = 2, nf = 6, G = 4) a generated kernel's
+// row takes 20 KB: 8 rows per CU.
 struct RowiLayout {
   long early;  // complex offset of the FFT lines: rows written before the last transform
   long late;   // complex size of the rows written after it (inside the lines' span)
@@ -467,7 +471,7 @@ struct RowiLayout {
 };
 __host__ __device__ inline RowiLayout rowi_layout(int L, int Nx, int n, int nf, int G) {
   const int ng = nf > 0 ? (nf + G - 1) / G : 0;
-  const int nearly = n + (ng > 1 ? (ng - 1) * G : 0);
+  const int nearly = n + (ng > 1 ? nf - G : 0);
   RowiLayout o;
   o.early = (long)nearly * Nx / 2;  // Nx is even
   o.late = (long)(n + nf - nearly) * Nx / 2;
@@ -475,9 +479,10 @@ __host__ __device__ inline RowiLayout rowi_layout(int L, int Nx, int n, int nf, 
   o.tail = lines > rest ? lines : rest;
   return o;
 }
-__host__ __device__ inline long rowi_region(int L, int Nx, int n, int nf, int nwords, int G) {
+__host__ __device__ inline long rowi_region(int L, int Nx, int n, int nf, int nwords, int G,
+                                            bool prog = true) {
   const RowiLayout o = rowi_layout(L, Nx, n, nf, G);
-  return o.early + o.tail + (nwords + 3) / 4;
+  return o.early + o.tail + (prog ? (nwords + 3) / 4 : 0);
 }
 
 // One grid row r, by the nt threads tid = 0..nt-1 of a row group whose LDS
@@ -520,9 +525,12 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   // computed by every line group, without a divergent branch, and kept by
   // those with a factor), the program and the identity rows.
   constexpr bool kPre = E <= 8;  // register budget: only for 8 values per thread
+  // factor groups (rowi_layout): the first holds rem factors, the others G
+  const int ng = nf > 0 ? (nf + G - 1) / G : 0;
+  const int rem = nf - (ng > 0 ? (ng - 1) * G : 0);
   cx v0[kPre ? E : 1];
   if constexpr (kPre) {
-    if (nf > 0) c2r_pre<E, LS, GK>(g, fl, b, twN, gl < nf ? gl : 0, r, t, T, v0);
+    if (nf > 0) c2r_pre<E, LS, GK>(g, fl, b, twN, gl < rem ? gl : 0, r, t, T, v0);
   }
   // With TWR (a compile-time line length, 8 values per thread) the twiddles
   // of the passes are held in registers, loaded here in the same round trip
@@ -559,9 +567,9 @@ __device__ __forceinline__ void rowi_body(long r, const Grid &g, const FftPlan &
   }
   for (int w = tid + IDU * nt; w < n * Nx; w += nt) vals[w] = src[(w / Nx) * N + r * Nx + (w % Nx)];
 
-  for (int g0 = 0; g0 < nf; g0 += G) {
+  for (int g0 = 0, gsz = rem; g0 < nf; g0 += gsz, gsz = G) {
     const int m = g0 + gl;
-    const bool act = m < nf;
+    const bool act = gl < gsz;
     cx v[E];
     if (act && kPre && g0 == 0) {
 #pragma unroll
@@ -754,7 +762,7 @@ __device__ __forceinline__ void rowi_rows(const Grid &g, const FftPlan &px,
   if constexpr (E <= 8) {
     const int Nx = GK::Nx ? GK::Nx : g.Nx, n = GK::n ? GK::n : g.n;
     const int nf = GK::nf >= 0 ? GK::nf : fl.nf;
-    const long region = rowi_region(L, Nx, n, nf, dict.nwords, G);
+    const long region = rowi_region(L, Nx, n, nf, dict.nwords, G, DE::kProg);
     const int gt = G * (L / E), sub = threadIdx.x / gt;
     rowi_body<E, DE, LS, GK, TWR, LEAN>((long)blockIdx.x * (blockDim.x / gt) + sub, g, px, twN, st, b, fl,
                              dict, src_kind, src_ptr, dst_kind, dst_slot, dst_ptr, err, ec, nx, G,

@@ -488,8 +488,10 @@ void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups
     go_col_l<E, false>(c, g, dim3(cg.tc * T), lds, p, cg, check, fl, mode, fpw);
 }
 
+// (a generated kernel keeps no program in LDS)
 size_t rowi_lds(const rdcu_ctx *c, int G, int R) {
-  return (size_t)R * rd::rowi_region(c->L, c->Nx, c->n, c->nf_used, c->dictp.nwords, G) *
+  return (size_t)R *
+         rd::rowi_region(c->L, c->Nx, c->n, c->nf_used, c->dictp.nwords, G, !c->jit_rowi) *
          sizeof(cx);
 }
 
