@@ -262,11 +262,15 @@ __global__ __launch_bounds__(256) void k_rowf_reg(Grid g, FftPlan px, const cx *
 // buf[pad8(e)*tc + c]) + tc*L complex holding the forward spectrum (mode 1).
 // TG: twiddles and axis-0 wavenumbers read from global memory instead of LDS
 // copies (long lines, where the copies would leave room for one column).
-template <int E, bool NT = false, int LS = 0, bool TG = false>
+// FO: built for modes 0 and 2 only (the transforms of the 3-D y passes): the
+// multiply path and its registers are compiled out, so that more blocks fit a
+// CU (the runtime launches it for the y forward pass).
+template <int E, bool NT = false, int LS = 0, bool TG = false, bool FO = false>
 __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan &p,
                                          const ColGeom &cg, const Bufs &b, const FacList &fl,
-                                         int mode, int fpw, const DevState *chk = nullptr) {
+                                         int mode_arg, int fpw, const DevState *chk = nullptr) {
   extern __shared__ cx smem[];
+  const int mode = FO ? (mode_arg == 0 ? 0 : 2) : mode_arg;
   // mode 1: blocks come in groups of nparts, each transforming up to fpw of
   // the field's factors (the forward transform is repeated per part)
   const int nparts = mode == 1 ? cg.nparts : 1;
@@ -340,7 +344,7 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     if (mode == 1) pf_k.store(k0l, g.kt[0], L, tid, blockDim.x);
   }
   cx *lb = buf + c;
-  if (mode != 1) {
+  if (FO || mode != 1) {
     if (mode == 0) fft_any<E, -1, LS>(v, lb, tc, tw, p, t, T);
     else fft_any<E, +1, LS>(v, lb, tc, tw, p, t, T);
     if (act) {
@@ -396,13 +400,13 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
 template <int E, int LS = 0>
 constexpr int col_max_threads() { return E == 8 && (LS == 0 || (LS > 0 && LS % 3 != 0)) ? 512 : 256; }
 
-template <int E, bool NT, int LS = 0, bool TG = false>
+template <int E, bool NT, int LS = 0, bool TG = false, bool FO = false>
 __global__ __launch_bounds__((col_max_threads<E, LS>())) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
                                                  int check_active, Bufs b, FacList fl, int mode,
                                                  int fpw) {
   const int lblk = xcd_block(blockIdx.x, gridDim.x);
   if (lblk >= cg.nblk) return;
-  col_body<E, NT, LS, TG>(lblk, g, p, cg, b, fl, mode, fpw, check_active ? st : nullptr);
+  col_body<E, NT, LS, TG, FO>(lblk, g, p, cg, b, fl, mode, fpw, check_active ? st : nullptr);
 }
 
 // Inverse x pass for one row position r, all fields: factor lines in groups
