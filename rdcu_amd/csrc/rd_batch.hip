@@ -83,6 +83,7 @@ namespace {
 
 int ninst(const rdcu_batch *b) { return (int)b->members.size(); }
 
+
 // LDS row region of the class (largest factor count and program).
 long rowi_region_b(const BatchClass &cl, int G) {
   const rdcu_ctx *c = cl.c0;
@@ -101,6 +102,23 @@ constexpr int kColLsB[] = {64, 128, 256};
 // strided lines 96 and 192)
 constexpr int kRowLsB3[] = {48, 96};
 constexpr int kColLsB3[] = {96, 192};
+
+// Raises the dynamic-LDS limit of every batched kernel to the 160 KB of a
+// gfx950 CU (for the device current when it runs).
+int allow_big_lds_b() {
+  const int lim = 160 * 1024;
+  for (const void *f :
+       {(const void *)rd::k_rowf_b<8>, (const void *)rd::k_rowf_b<8, -1>,
+        (const void *)rd::k_col_b<8>, (const void *)rd::k_col_b<8, -1>,
+        (const void *)rd::k_rowi_b<8>, (const void *)rd::k_rowi_b<8, -1>,
+        (const void *)rd::k_rowf_b<8, kRowLsB[0]>, (const void *)rd::k_rowf_b<8, kRowLsB[1]>,
+        (const void *)rd::k_rowf_b<8, kRowLsB[2]>, (const void *)rd::k_col_b<8, kColLsB[0]>,
+        (const void *)rd::k_col_b<8, kColLsB[1]>, (const void *)rd::k_col_b<8, kColLsB[2]>,
+        (const void *)rd::k_rowf_b<8, kRowLsB3[0]>, (const void *)rd::k_rowf_b<8, kRowLsB3[1]>,
+        (const void *)rd::k_col_b<8, kColLsB3[0]>, (const void *)rd::k_col_b<8, kColLsB3[1]>})
+    HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  return RDCU_OK;
+}
 
 template <int E, int LS>
 void go_rowf_b(rdcu_batch *b, const BatchClass &cl, dim3 g, dim3 bl, size_t lds, int per,
@@ -191,6 +209,8 @@ void launch_rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_ki
       int a_src = src_kind, a_dk = dst_kind, a_ds = dst_slot, a_err = err, a_G = G, a_chk = check;
       void *args[] = {&in, &cnt, &rows, &px, &tw, &a_src, &a_dk, &a_ds, &a_err, &ec, &nx, &a_G,
                       &a_chk, &region};
+      // a failed launch is reported by the hipGetLastError check after each
+      // group of launches (and inside a graph capture it fails the capture)
       (void)hipModuleLaunchKernel(gr.fn, grid, 1, 1, block, 1, 1, (unsigned)lds, b->ls, args,
                                   nullptr);
     } else if (odd_len(c->L)) {
@@ -639,24 +659,9 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
     b->d_inst = static_cast<rd::Inst *>(q);
     if ((st = build_table(b.get(), std::vector<char>(count, 1)))) return st;
   }
-  {
-    static bool lds_raised = false;
-    if (!lds_raised) {
-      const int lim = 160 * 1024;
-      for (const void *f :
-           {(const void *)rd::k_rowf_b<8>, (const void *)rd::k_rowf_b<8, -1>,
-            (const void *)rd::k_col_b<8>, (const void *)rd::k_col_b<8, -1>,
-            (const void *)rd::k_rowi_b<8>, (const void *)rd::k_rowi_b<8, -1>,
-            (const void *)rd::k_rowf_b<8, kRowLsB[0]>, (const void *)rd::k_rowf_b<8, kRowLsB[1]>,
-            (const void *)rd::k_rowf_b<8, kRowLsB[2]>, (const void *)rd::k_col_b<8, kColLsB[0]>,
-            (const void *)rd::k_col_b<8, kColLsB[1]>, (const void *)rd::k_col_b<8, kColLsB[2]>,
-            (const void *)rd::k_rowf_b<8, kRowLsB3[0]>, (const void *)rd::k_rowf_b<8, kRowLsB3[1]>,
-            (const void *)rd::k_col_b<8, kColLsB3[0]>, (const void *)rd::k_col_b<8, kColLsB3[1]>})
-        HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-      if ((st = allow_big_lds())) return st;
-      lds_raised = true;
-    }
-  }
+  if ((st = once_per_device(b->dev, 1, allow_big_lds_b)) ||
+      (st = once_per_device(b->dev, 0, allow_big_lds)))
+    return st;
   for (BatchClass &cl : b->classes) {
     cl.c0 = b->members[cl.members[0]].get();
     cl.Ex = cl.c0->Ex;

@@ -547,6 +547,8 @@ void launch_rowi(rdcu_ctx *c, int check, int src_kind, const double *src_ptr, in
     void *args[] = {&a_g, &a_px, &a_tw, &a_st, &a_check, &a_b, &a_fl, &a_d, &a_src,
                     &a_sp, &a_dk, &a_ds, &a_dp, &a_err, &a_ec, &a_nx, &a_G};
     hipFunction_t fn = jit_rowi_variant(c, c->ri_tw);
+    // a failed launch is reported by the hipGetLastError check after each
+    // group of launches (and inside a graph capture it fails the capture)
     (void)hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, (unsigned)lds, c->stream, args,
                                 nullptr);
     return;
@@ -1154,6 +1156,22 @@ int allow_big_lds() {
   return RDCU_OK;
 }
 
+// Runs fn (a set-up such as raising kernels' dynamic-LDS limit, which holds
+// for the device current when it runs) once per device and slot, whichever
+// thread gets there first; calls that arrive meanwhile wait for it.  A failure
+// is returned to the caller and the set-up is tried again next time.
+int once_per_device(int dev, int slot, int (*fn)()) {
+  static std::mutex mu;
+  static bool done[kMaxDevices][2] = {};
+  if (dev < 0 || dev >= kMaxDevices || slot < 0 || slot > 1)
+    return fail(RDCU_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (done[dev][slot]) return RDCU_OK;
+  const int st = fn();
+  if (st == RDCU_OK) done[dev][slot] = true;
+  return st;
+}
+
 // Every factor of fl (every derivative block of every field: fl_all, or
 // their imaginary parts: fl_imag), real space, into out (S must hold the
 // current forward spectra; it is only read).
@@ -1620,11 +1638,7 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
   }
 
   if (c->reg) {
-    static bool lds_raised = false;
-    if (!lds_raised) {
-      if ((st = allow_big_lds())) return st;
-      lds_raised = true;
-    }
+    if ((st = once_per_device(device, 0, allow_big_lds))) return st;
     build_jit(c.get());
     const char *shp = getenv("RDCU_SHAPES");  // "rpb,tc,fpw,G": fixed shapes, no tuning
     if (shp && *shp) {
