@@ -801,8 +801,11 @@ __device__ __forceinline__ void attempt1d_body(const Grid &g, const FftPlan &px,
   const int nt = blockDim.x, G = nt / T;
   rowf_body<E, false, LS>(0, g, px, twN, st, b, SRC_STAGE, nullptr, ac.sc2, G);
   __syncthreads();
+  // the phases' transforms take their twiddles from registers, loaded with
+  // each phase's first loads: in one workgroup nothing hides a read of the
+  // table in every pass
   for (int s = 2; s <= 7; ++s) {
-    rowi_body<E, DE, LS, GK>(0, g, px, twN, st, b, fl, dict, SRC_YT, nullptr,
+    rowi_body<E, DE, LS, GK, true>(0, g, px, twN, st, b, fl, dict, SRC_YT, nullptr,
                              s < 7 ? DST_KS : DST_K7, s - 1, nullptr, s == 7 && !fixed, ac.ec,
                              ac.nx[s - 2], G, smem, threadIdx.x, nt);
     __syncthreads();
