@@ -17,6 +17,9 @@ struct BatchGroup {  // instances of one class sharing a dictionary structure
   std::vector<int> all;  // its members
   int i0 = 0, i1 = 0;    // range in the class's current table
   hipFunction_t fn = nullptr;  // specialised x-inverse kernel (null: interpreter)
+  // its variants: 1 twiddles in registers, 2 lean, 3 lean with the twiddles
+  // in registers (null: not built)
+  hipFunction_t fnv[4] = {};
 };
 
 struct BatchClass {
@@ -27,6 +30,7 @@ struct BatchClass {
   rdcu_ctx *c0 = nullptr;    // a representative member (geometry, plans, tables)
   int Ex = 0, E0 = 0, rows = 0, nf_max = 0, nwords_max = 0, max_fpf = 1;
   int rpb = 1, tc = 1, fpw = 1, G = 1, R = 1;
+  int rv = 0;        // x-inverse variant (BatchGroup::fnv), the tuner's choice
   long grid_rf = 0;  // persistent grid of the x-forward pass (0: one block per item)
   std::string tune_log;
 };
@@ -203,7 +207,8 @@ void launch_rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_ki
     if (cnt == 0) continue;
     const unsigned grid = (unsigned)((long)cnt * rows / R);  // R rows per block, R | rows
     const unsigned block = (unsigned)(R * G * T);
-    if (gr.fn) {
+    hipFunction_t fn = cl.rv > 0 && gr.fnv[cl.rv] ? gr.fnv[cl.rv] : gr.fn;
+    if (fn) {
       rd::FftPlan px = c->px;
       const rd::cx *tw = c->twN;
       int a_src = src_kind, a_dk = dst_kind, a_ds = dst_slot, a_err = err, a_G = G, a_chk = check;
@@ -211,7 +216,7 @@ void launch_rowi_b(rdcu_batch *b, const BatchClass &cl, int src_kind, int dst_ki
                       &a_chk, &region};
       // a failed launch is reported by the hipGetLastError check after each
       // group of launches (and inside a graph capture it fails the capture)
-      (void)hipModuleLaunchKernel(gr.fn, grid, 1, 1, block, 1, 1, (unsigned)lds, b->ls, args,
+      (void)hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, (unsigned)lds, b->ls, args,
                                   nullptr);
     } else if (odd_len(c->L)) {
       hipLaunchKernelGGL((rd::k_rowi_b<E, -1>), dim3(grid), dim3(block), lds, b->ls, in, cnt,
@@ -346,19 +351,32 @@ int autotune_b(rdcu_batch *b, BatchClass &cl) {
   // idle lines, but more threads per row: at short rows G*T is a fraction of
   // a wave)
   const int Gf = std::max(c->n, cl.nf_max);
-  for (int G = c->n; G * T <= 256; G = G < Gf ? G + 1 : (G == Gf ? pow2_above(Gf) : 2 * G)) {
-    for (int R = 1; R * G * T <= 256 && c->rows % R == 0 && (R == 1 || cl.Ex <= 8); R *= 2) {
-      if (rowi_lds(cl, G, R) > 160 * 1024) break;
-      double tm = time_of([&]() { rowi_b(b, cl, rd::SRC_Y, rd::DST_KS, 1, 0, none, G, R, 0); });
-      snprintf(line, sizeof line, "row_inv G=%d R=%d %.1fus; ", G, R, tm * 1e3);
-      cl.tune_log += line;
-      if (tm < best) {
-        best = tm;
-        cl.G = G;
-        cl.R = R;
+  // and the generated kernel's variants (those built for every group;
+  // RDCU_BATCH_XINV=v times only variant v, for tests)
+  int best_rv = 0;
+  const char *xv = getenv("RDCU_BATCH_XINV");
+  const int only = xv && *xv ? atoi(xv) : -1;
+  for (int rv = 0; rv < 4; ++rv) {
+    bool all = true;
+    for (const BatchGroup &gr : cl.groups) all = all && gr.fn && (rv == 0 || gr.fnv[rv]);
+    if ((rv > 0 && !all) || (only >= 0 && rv != only)) continue;
+    cl.rv = rv;
+    for (int G = c->n; G * T <= 256; G = G < Gf ? G + 1 : (G == Gf ? pow2_above(Gf) : 2 * G)) {
+      for (int R = 1; R * G * T <= 256 && c->rows % R == 0 && (R == 1 || cl.Ex <= 8); R *= 2) {
+        if (rowi_lds(cl, G, R) > 160 * 1024) break;
+        double tm = time_of([&]() { rowi_b(b, cl, rd::SRC_Y, rd::DST_KS, 1, 0, none, G, R, 0); });
+        snprintf(line, sizeof line, "row_inv v=%d G=%d R=%d %.1fus; ", rv, G, R, tm * 1e3);
+        cl.tune_log += line;
+        if (tm < best) {
+          best = tm;
+          cl.G = G;
+          cl.R = R;
+          best_rv = rv;
+        }
       }
     }
   }
+  cl.rv = best_rv;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   HIPCHECK(hipGetLastError());
@@ -387,14 +405,22 @@ void build_jit_b(rdcu_batch *b) {
                "Nx = %d, Nxp = %d, nf = %d;\n  static constexpr long N = %ldL, rows = %ldL;\n};\n}\n",
                tag.c_str(), gd.n, gd.ndim, gd.Nx, gd.Nxp, m->fl_used.nf, gd.N, gd.rows);
       src += gbuf;
-      src += "extern \"C\" __global__ __launch_bounds__(256) void rdcu_rowi_b_" + tag +
-             "(const rd::Inst *__restrict__ in, int ninst, int rows, rd::FftPlan px, const rd::cx "
-             "*__restrict__ twN, int src_kind, int dst_kind, int dst_slot, int err, rd::ErrCoef "
-             "ec, rd::StageCoef nx, int G, int check, long region) {\n";
-      src += "  rd::rowi_batch<" + std::to_string(cl.Ex) + ", rd::JitDict_" + tag + ", " +
-             std::to_string(m->L) + ", rd::JitGeom_" + tag +
-             ">(in, ninst, rows, px, twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check, "
-             "region);\n}\n";
+      // the kernel, and (8 values per thread) its variants with the twiddles
+      // in registers and compiled lean for two wavefronts per SIMD
+      static const struct { const char *sfx, *bounds, *args; } vs[] = {
+          {"", "256", ""}, {"_v1", "256", ", true, false"},
+          {"_v2", "256, 2", ", false, true"}, {"_v3", "256, 2", ", true, true"}};
+      for (int v = 0; v < (cl.Ex == 8 ? 4 : 1); ++v) {
+        src += std::string("extern \"C\" __global__ __launch_bounds__(") + vs[v].bounds +
+               ") void rdcu_rowi_b_" + tag + vs[v].sfx +
+               "(const rd::Inst *__restrict__ in, int ninst, int rows, rd::FftPlan px, const "
+               "rd::cx *__restrict__ twN, int src_kind, int dst_kind, int dst_slot, int err, "
+               "rd::ErrCoef ec, rd::StageCoef nx, int G, int check, long region) {\n";
+        src += "  rd::rowi_batch<" + std::to_string(cl.Ex) + ", rd::JitDict_" + tag + ", " +
+               std::to_string(m->L) + ", rd::JitGeom_" + tag + vs[v].args +
+               ">(in, ninst, rows, px, twN, src_kind, dst_kind, dst_slot, err, ec, nx, G, check, "
+               "region);\n}\n";
+      }
     }
   }
   std::vector<char> code;
@@ -409,6 +435,11 @@ void build_jit_b(rdcu_batch *b) {
     return;
   }
   b->jit_mod = mod;
+  // kernels that fail to load or spill are left out above; their lookup
+  // errors are not kept as the thread's last error
+  struct ClearLast {
+    ~ClearLast() { (void)hipGetLastError(); }
+  } clear_last;
   for (size_t ci = 0; ci < b->classes.size(); ++ci) {
     BatchClass &cl = b->classes[ci];
     for (size_t gi = 0; gi < cl.groups.size(); ++gi) {
@@ -416,6 +447,13 @@ void build_jit_b(rdcu_batch *b) {
       hipFunction_t f = nullptr;
       if (hipModuleGetFunction(&f, mod, nm.c_str()) == hipSuccess && !jit_spills(f))
         cl.groups[gi].fn = f;
+      // a variant that does not load or would spill is left out
+      for (int v = 1; cl.groups[gi].fn && cl.Ex == 8 && v < 4; ++v) {
+        const std::string nv = nm + "_v" + std::to_string(v);
+        f = nullptr;
+        if (hipModuleGetFunction(&f, mod, nv.c_str()) == hipSuccess && !jit_spills(f))
+          cl.groups[gi].fnv[v] = f;
+      }
     }
   }
   b->jit_log = "ok";
@@ -696,9 +734,9 @@ int rdcu_batch_create(const rdcu_problem *probs, int count, const double *const 
     const char *vb = getenv("RDCU_VERBOSE");
     if (vb && *vb && *vb != '0')
       fprintf(stderr, "rdcu batch class %dx%d (%zu instances, %zu structures, jit %s): %s -> "
-                      "%d,%d,%d,%d,%d x-forward grid %ld\n",
+                      "%d,%d,%d,%d,%d x-inverse variant %d x-forward grid %ld\n",
               cl.c0->Ns[0], cl.c0->Ns[1], cl.members.size(), cl.groups.size(),
-              b->jit_log.c_str(), cl.tune_log.c_str(), cl.rpb, cl.tc, cl.fpw, cl.G, cl.R,
+              b->jit_log.c_str(), cl.tune_log.c_str(), cl.rpb, cl.tc, cl.fpw, cl.G, cl.R, cl.rv,
               cl.grid_rf);
   }
   for (size_t c = 0; c < b->classes.size(); ++c) {  // one stream per class (default priority)

@@ -866,8 +866,8 @@ __global__ __launch_bounds__(256) void k_col_b(const Inst *__restrict__ in, int 
 
 // The x-inverse pass takes one (instance, row) item per block: it is the
 // kernel with the most live registers, and a grid-stride loop around it
-// would cost more.
-template <int E, class DE, int LS = 0, class GK = RunGeom>
+// would cost more.  TWR and LEAN as for rowi_body (generated kernels).
+template <int E, class DE, int LS = 0, class GK = RunGeom, bool TWR = false, bool LEAN = false>
 __device__ __forceinline__ void rowi_batch(const Inst *__restrict__ in, int ninst, int rows,
                                            const FftPlan &px, const cx *__restrict__ twN,
                                            int src_kind, int dst_kind, int dst_slot, int err,
@@ -883,9 +883,9 @@ __device__ __forceinline__ void rowi_batch(const Inst *__restrict__ in, int nins
     DevState *st = in[i].st;
     if (check && !attempt_active(st)) return;
     const Inst &I = in[i];
-    rowi_body<E, DE, LS, GK>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind,
-                             nullptr, dst_kind, dst_slot, nullptr, err, ec, nx, G,
-                             smem + sub * region, threadIdx.x - sub * gt, gt);
+    rowi_body<E, DE, LS, GK, TWR, LEAN>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict,
+                                        src_kind, nullptr, dst_kind, dst_slot, nullptr, err, ec,
+                                        nx, G, smem + sub * region, threadIdx.x - sub * gt, gt);
   } else {  // one row per block (see rowi_rows)
     const long it = blockIdx.x;
     if (it >= (long)ninst * rows) return;
@@ -893,9 +893,9 @@ __device__ __forceinline__ void rowi_batch(const Inst *__restrict__ in, int nins
     DevState *st = in[i].st;
     if (check && !attempt_active(st)) return;
     const Inst &I = in[i];
-    rowi_body<E, DE, LS, GK>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict, src_kind,
-                             nullptr, dst_kind, dst_slot, nullptr, err, ec, nx, G, smem,
-                             threadIdx.x, blockDim.x);
+    rowi_body<E, DE, LS, GK, TWR, LEAN>(it - (long)i * rows, I.g, px, twN, st, I.b, I.fl, I.dict,
+                                        src_kind, nullptr, dst_kind, dst_slot, nullptr, err, ec,
+                                        nx, G, smem, threadIdx.x, blockDim.x);
   }
 }
 

@@ -19,11 +19,17 @@ def mixed_set():
             Problem(2, [256], [60.0], cgle_terms(1, 2.0, 0.8))]
 
 
-def test_batch_steps_bitwise_vs_oracle(hlib, olib):
+@pytest.mark.parametrize("variant", [None, 0, 1, 2, 3],
+                         ids=["tuned", "xinv-plain", "xinv-twreg", "xinv-lean", "xinv-lean-twreg"])
+def test_batch_steps_bitwise_vs_oracle(hlib, olib, monkeypatch, variant):
+    """With the tuner's choice, and with each generated x-inverse variant
+    (the tuner restricted to it)."""
+    if variant is not None:
+        monkeypatch.setenv("RDCU_BATCH_XINV", str(variant))
     probs = mixed_set()
     ys = [glibc_rand_ic(2, p.N, seed=1000 + i, A=1.0) for i, p in enumerate(probs)]
     n = len(probs)
-    print("case batch steps", flush=True)
+    print(f"case batch steps, x-inverse variant {variant}", flush=True)
     with Batch(hlib, probs, ys) as g, Batch(olib, probs, ys) as o:
         tg, hg = np.zeros(n), np.full(n, 0.01)
         to, ho = tg.copy(), hg.copy()
