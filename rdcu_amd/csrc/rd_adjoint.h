@@ -256,6 +256,56 @@ __global__ void k_stage_in(const double *__restrict__ y, StageIn si, const doubl
   out[i] = __builtin_fma(hs[k], sacc, y[i]);
 }
 
+struct AdjK {
+  double *k[7];
+  double c[7];  // per-array coefficient (0: skip)
+};
+
+// The start of a reverse step when the forward pass stored every step's stage
+// values (kst: stage j of step k at kst + (7k + j) nN), in one pass over the
+// points instead of nine launches: ycur = the checkpoint of step k; the inputs
+// of stages 2..7 (out[s-2]) with the operations of k_stage_in; the stage
+// adjoints and ybar set to 0 and the dense outputs' adjoints added with the
+// operations of k_adj_zero and k_adj_dense.
+struct StageIn6 {
+  double a[6][6];     // a[s-2][j]: the tableau row of stage s (j < s-1)
+  double *out[6];     // the inputs of stages 2..7
+};
+__global__ void k_adj_begin(const double *__restrict__ ych, const long long *__restrict__ kp,
+                            double *__restrict__ ycur, const double *__restrict__ kst,
+                            const double *__restrict__ hs, StageIn6 si, double *ybar, AdjK kb,
+                            const double *__restrict__ g, const int *__restrict__ obeg,
+                            const int *__restrict__ oend, const double *__restrict__ oc,
+                            long nN) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nN) return;
+  const long long k = *kp;
+  const double y = ych[(long)k * nN + i];
+  ycur[i] = y;
+  const long off = (long)k * 7 * nN + i;
+  const double hk = hs[k];
+#pragma unroll
+  for (int s = 2; s <= 7; ++s) {
+    double sacc = si.a[s - 2][0] * kst[off];
+    for (int j = 1; j < s - 1; ++j)
+      if (si.a[s - 2][j] != 0.0) sacc = __builtin_fma(si.a[s - 2][j], kst[off + (long)j * nN], sacc);
+    si.out[s - 2][i] = __builtin_fma(hk, sacc, y);
+  }
+  double yb = 0.0, kbv[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int o = oend[k] - 1; o >= obeg[k]; --o) {
+    const double gv = g[(long)o * nN + i];
+    yb += gv;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      const double cj = oc[7 * o + j];
+      if (cj != 0.0) kbv[j] += cj * gv;
+    }
+  }
+  ybar[i] = yb;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) kb.k[j][i] = kbv[j];
+}
+
 // ycur = checkpoint (state at the start) of step k.
 __global__ void k_adj_fetch(const double *__restrict__ ych, const long long *__restrict__ kp,
                             double *__restrict__ ycur, long nN) {
@@ -291,11 +341,6 @@ __global__ void k_loss(const double *__restrict__ y, const double *__restrict__ 
   }
   if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
-
-struct AdjK {
-  double *k[7];
-  double c[7];  // per-array coefficient (0: skip)
-};
 
 // Zeroes the stage adjoints kbar_0..6 (kb.k) and ybar.
 __global__ void k_adj_zero(AdjK kb, double *ybar, long nN) {

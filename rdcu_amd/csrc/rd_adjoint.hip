@@ -572,10 +572,50 @@ int loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const doub
   HIPCHECK(hipMemcpyAsync(d_oend, oend.data(), ns * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemcpyAsync(d_oc, oc.data(), oc.size() * sizeof(double), hipMemcpyHostToDevice,
                           c->stream));
+  // The rest of a reverse step: the stage adjoints in reverse stage order
+  // (vector-Jacobian products and their accumulation), then the step index.
+  auto enqueue_step_tail = [&](double *lam_in, double *ybar_out) -> int {
+    const unsigned nb = blocks_of(nN);
+    int s2;
+    for (int s = 7; s >= 1; --s) {
+      const double *u = s == 1 ? ycur : Ys[s - 1];
+      rd::Bufs pre = c->bufs;
+      pre.S = S7 + (size_t)(s - 1) * c->n * c->Nh;
+      pre.F = F7 + (size_t)(s - 1) * nf * c->Nh;
+      if ((s2 = vjp(c, u, kbar[s - 1], out_id, Gk, Dk, P, tmap, batched_spectra ? &pre : nullptr)))
+        return s2;
+      rd::AdjK kb{};
+      for (int j = 0; j < 7; ++j) kb.k[j] = kbar[j];
+      if (s >= 2)
+        for (int j = 0; j < s - 1; ++j) kb.c[j] = STAGE_A[s - 2][j];
+      hipLaunchKernelGGL(rd::k_adj_stage, dim3(nb), dim3(256), 0, c->stream,
+                         (const double *)out_id, (const double *)Dk, (const int *)fmap, nf,
+                         (const double *)(s == 7 ? lam_in : nullptr), ybar_out, kb,
+                         (const double *)d_hs, (const long long *)kdev, N, nN);
+    }
+    hipLaunchKernelGGL(rd::k_adj_next, dim3(1), dim3(64), 0, c->stream, kdev);
+    HIPCHECK(hipGetLastError());
+    return RDCU_OK;
+  };
   // One reverse step (the step index from kdev), with lam_in the adjoint of
   // the step's end state and ybar_out that of its start.
   auto enqueue_step = [&](double *lam_in, double *ybar_out) -> int {
     const unsigned nb = blocks_of(nN);
+    if (kst) {  // stored stages: the step's point-wise start in one kernel
+      rd::StageIn6 si6{};
+      for (int s = 2; s <= 7; ++s) {
+        for (int j = 0; j < s - 1; ++j) si6.a[s - 2][j] = STAGE_A[s - 2][j];
+        si6.out[s - 2] = Ys[s - 1];
+      }
+      rd::AdjK kz{};
+      for (int j = 0; j < 7; ++j) kz.k[j] = kbar[j];
+      hipLaunchKernelGGL(rd::k_adj_begin, dim3(nb), dim3(256), 0, c->stream, ych,
+                         (const long long *)kdev, ycur, (const double *)kst, (const double *)d_hs,
+                         si6, ybar_out, kz, (const double *)gbar, (const int *)d_obeg,
+                         (const int *)d_oend, (const double *)d_oc, nN);
+      if (batched_spectra) enqueue_spectra7();
+      return enqueue_step_tail(lam_in, ybar_out);
+    }
     hipLaunchKernelGGL(rd::k_adj_fetch, dim3(nb), dim3(256), 0, c->stream, ych, kdev, ycur, nN);
     // stage inputs from the checkpoint (the forward's operations); the stage
     // values stored by the forward pass, or recomputed
@@ -601,25 +641,7 @@ int loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const doub
     hipLaunchKernelGGL(rd::k_adj_dense, dim3(nb), dim3(256), 0, c->stream, ybar_out, kz,
                        (const double *)gbar, (const int *)d_obeg, (const int *)d_oend,
                        (const double *)d_oc, (const long long *)kdev, nN);
-    for (int s = 7; s >= 1; --s) {
-      const double *u = s == 1 ? ycur : Ys[s - 1];
-      rd::Bufs pre = c->bufs;
-      pre.S = S7 + (size_t)(s - 1) * c->n * c->Nh;
-      pre.F = F7 + (size_t)(s - 1) * nf * c->Nh;
-      if ((s2 = vjp(c, u, kbar[s - 1], out_id, Gk, Dk, P, tmap, batched_spectra ? &pre : nullptr)))
-        return s2;
-      rd::AdjK kb{};
-      for (int j = 0; j < 7; ++j) kb.k[j] = kbar[j];
-      if (s >= 2)
-        for (int j = 0; j < s - 1; ++j) kb.c[j] = STAGE_A[s - 2][j];
-      hipLaunchKernelGGL(rd::k_adj_stage, dim3(nb), dim3(256), 0, c->stream,
-                         (const double *)out_id, (const double *)Dk, (const int *)fmap, nf,
-                         (const double *)(s == 7 ? lam_in : nullptr), ybar_out, kb,
-                         (const double *)d_hs, (const long long *)kdev, N, nN);
-    }
-    hipLaunchKernelGGL(rd::k_adj_next, dim3(1), dim3(64), 0, c->stream, kdev);
-    HIPCHECK(hipGetLastError());
-    return RDCU_OK;
+    return enqueue_step_tail(lam_in, ybar_out);
   };
   // The step is captured twice, once per role of the two adjoint buffers
   // (they swap every step), and replayed ns times; x[2] holds kGraphAttempts
