@@ -48,6 +48,36 @@ def algorithmic_bytes(n, M, Ns):
     return out
 
 
+def attempt_bytes(n, M, Ns):
+    """Algorithmic HBM bytes of one DP45 attempt through the fused kernels,
+    one read and one write per array per kernel (A = 8nN: a state-size array
+    or the n half spectra; F = 8MN: the M factor spectra):
+      x forward + stage-2 sum        read y, k1; write ytemp2, S         4A
+      per RHS (stages 2..7)          strided: read S, write F           A + F
+                                     (3-D: y forward 2A, y inverse 2F)
+      x inverse of stage s = 2..6    read F, ytemp_s, y and the k_j (j < s) of
+                                     stage s+1's sum with nonzero a_{s+1,j}
+                                     (k_s stays on chip); write k_s,
+                                     ytemp_{s+1}, S
+      x inverse of stage 7           read F, ytemp7, y and the k_j (j < 7) with
+                                     a nonzero error weight; write k7
+    The zero pattern is DP45's: a_72 = b_2 = 0 and e_2 = 0 (dp45.cu:11-13);
+    every other a_ij (j < i) and e_j is nonzero.  The step control's error
+    partials (one double per line) are left out."""
+    N = 1
+    for s in Ns:
+        N *= s
+    A, F = 8 * n * N, 8 * M * N
+    nz_sum = {2: 1, 3: 2, 4: 3, 5: 4, 6: 4}  # k_j read for stage s+1's sum, j < s
+    nz_err = 5                                # e_1, e_3..e_6
+    rhs = A + F + ((2 * A + 2 * F) if len(Ns) == 3 else 0)
+    b = 4 * A + 6 * rhs
+    for s in range(2, 7):
+        b += F + (2 + nz_sum[s]) * A + 3 * A
+    b += F + (2 + nz_err) * A + A
+    return b
+
+
 PROFILE = os.path.join(ROOT, "profiles", "rhs_2d_cgle_512.md")
 KERNEL_SYMBOL = {"row_fwd": "k_rowf_reg", "col_mult": "k_col_reg", "row_inv": "rdcu_rowi_jit",
                  "row_r2c": "k_row_r2c", "col": "k_col", "row_c2r": "k_row_c2r"}
@@ -469,6 +499,10 @@ def main():
                 "frac": rhs_ach / PEAK_HBM, "kernels_ms": km, "kernel_bytes": kb,
                 "rhs_per_s": 1e3 / km["rhs"]},
     }
+    B_att = attempt_bytes(2, M, Ns)
+    result["attempt"] = {"bytes": B_att, "us": el / attempts * 1e6,
+                         "achieved_GBps": B_att * attempts / el / 1e9,
+                         "frac": B_att * attempts / el / PEAK_HBM}
 
     if args.cpu_seconds > 0:
         result["cpu_baseline"] = cpu_leg(prob, y0, gname, args.warmup, attempts, accepted / el,
