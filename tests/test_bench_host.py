@@ -39,3 +39,23 @@ def test_measured_traffic_comes_from_the_counter_table():
     # within 10% of the kernel's algorithmic bytes (no re-reads)
     alg = bench.algorithmic_bytes(2, 4, [512, 512])["col_mult"]
     assert 1.0 <= bench.measured_traffic("col_mult") / alg < 1.1
+
+
+def test_attempt_bytes_match_the_per_kernel_listing():
+    """The whole-attempt byte model against the attempt profile's table, which
+    lists each kernel's arrays one by one (the strided row covers its six
+    launches): 339.7 MB at 512^2 (n=2, M=4)."""
+    path = os.path.join(ROOT, "profiles", "attempt_2d_cgle_512.md")
+    total = 0.0
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 8 and cells[0][:1].isdigit():
+            mb = float(cells[4])
+            total += 6 * mb if "," in cells[0] else mb
+    assert abs(total - 339.7) < 0.2
+    assert abs(bench.attempt_bytes(2, 4, [512, 512]) / 1e6 - total) < 0.2
+    # 3-D adds the y forward (2A) and y inverse (2F) passes to each of the six RHS
+    N = 256 ** 3
+    A, F = 16 * N, 48 * N
+    assert bench.attempt_bytes(2, 6, [256] * 3) - bench.attempt_bytes(2, 6, [256, 256 * 256]) \
+        == 6 * (2 * A + 2 * F)
