@@ -28,33 +28,43 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(n, M, Ns):
+def algorithmic_bytes(n, M, Ns, Mz=None):
     """Bytes one RHS evaluation must move through HBM, per kernel, for
     half-spectrum storage (8 B per grid point per field or factor) with one
     read and one write per array per pass:
       x forward   read u (8nN), write n half spectra (8nN)
-      y forward   (3-D) read and write the n spectra (16nN)
-      last axis   read n spectra (8nN), write M factor spectra (8MN)
-      y inverse   (3-D) read and write the M factor spectra (16MN)
+      y forward   (3-D) read and write the n spectra (16nN), and write the
+                  M - Mz factors without k_z, made in this pass (8(M-Mz)N)
+      last axis   read n spectra (8nN), write the factor spectra that go
+                  through it (2-D: all M; 3-D: the Mz with k_z) (8MN, 8MzN)
+      y inverse   (3-D) read and write the Mz factor spectra (16MzN)
       x inverse   read M factor spectra (8MN) and u (8nN), write f (8nN)
-    The sum is B_rhs = 8N[n(2*ndim+1) + 2M(ndim-1)]."""
+    Mz (3-D) defaults to n: the Laplacian's d2/dz2 of each field.  The sum is
+    8N[n(2 ndim + 1) + 2M] in 2-D and 8N[7n + 2M + 2Mz] in 3-D."""
     N = 1
     for s in Ns:
         N *= s
-    fwd, col, inv = 16 * n * N, 8 * n * N + 8 * M * N, 8 * M * N + 16 * n * N
-    out = {"row_r2c": fwd, "row_fwd": fwd, "col_mult": col, "row_c2r": inv, "row_inv": inv}
+    if Mz is None:
+        Mz = n
+    fwd, inv = 16 * n * N, 8 * M * N + 16 * n * N
+    out = {"row_r2c": fwd, "row_fwd": fwd, "row_c2r": inv, "row_inv": inv}
     if len(Ns) == 3:
-        out.update({"col_fwd": 16 * n * N, "col_inv": 16 * M * N})
+        out.update({"col_fwd": 16 * n * N + 8 * (M - Mz) * N, "col_mult": 8 * n * N + 8 * Mz * N,
+                    "col_inv": 16 * Mz * N})
+    else:
+        out["col_mult"] = 8 * n * N + 8 * M * N
     return out
 
 
-def attempt_bytes(n, M, Ns):
+def attempt_bytes(n, M, Ns, Mz=None):
     """Algorithmic HBM bytes of one DP45 attempt through the fused kernels,
     one read and one write per array per kernel (A = 8nN: a state-size array
     or the n half spectra; F = 8MN: the M factor spectra):
       x forward + stage-2 sum        read y, k1; write ytemp2, S         4A
       per RHS (stages 2..7)          strided: read S, write F           A + F
-                                     (3-D: y forward 2A, y inverse 2F)
+                                     (3-D: the y forward pass 2A and the
+                                     factors without k_z; z: A and the Mz
+                                     factors with k_z; y inverse 2 of those)
       x inverse of stage s = 2..6    read F, ytemp_s, y and the k_j (j < s) of
                                      stage s+1's sum with nonzero a_{s+1,j}
                                      (k_s stays on chip); write k_s,
@@ -68,9 +78,12 @@ def attempt_bytes(n, M, Ns):
     for s in Ns:
         N *= s
     A, F = 8 * n * N, 8 * M * N
+    if Mz is None:
+        Mz = n
+    Fz = 8 * Mz * N  # 3-D: the factors with k_z
     nz_sum = {2: 1, 3: 2, 4: 3, 5: 4, 6: 4}  # k_j read for stage s+1's sum, j < s
     nz_err = 5                                # e_1, e_3..e_6
-    rhs = A + F + ((2 * A + 2 * F) if len(Ns) == 3 else 0)
+    rhs = (3 * A + (F - Fz) + 3 * Fz) if len(Ns) == 3 else A + F
     b = 4 * A + 6 * rhs
     for s in range(2, 7):
         b += F + (2 + nz_sum[s]) * A + 3 * A

@@ -384,6 +384,7 @@ struct Problem {
   std::vector<Term> terms;
   std::vector<std::vector<double>> kt;  // wavenumbers per axis
   double invN = 0;
+  double invNxy = 0;  // 3-D: 1/(N_y N_x), the scale of the factors without k_z
 };
 
 int build_problem(const rdcu_problem *p, Problem &P) {
@@ -412,6 +413,7 @@ int build_problem(const rdcu_problem *p, Problem &P) {
   P.Nh = P.rows * P.Nxh;
   P.nblk = 1 + P.ndim + P.ndim * P.ndim;
   P.invN = 1.0 / (double)P.N;
+  P.invNxy = P.ndim == 3 ? 1.0 / (double)((long)P.Ns[1] * P.Ns[2]) : P.invN;
   P.kt.assign(P.ndim, {});
   for (int a = 0; a < P.ndim; ++a) {
     int Na = P.Ns[a];
@@ -459,7 +461,9 @@ inline double ipow(double v, int e) {
 // Nyquist plane of axis a for first derivatives, -k_a*k_b where exactly one of
 // a, b is at Nyquist for mixed ones, zero elsewhere.
 constexpr int kImagBlk = 64;
-inline cx apply_mult(const Problem &P, int blk, const int *nk, cx X) {
+// sc: the scale, 1/N (P.invN), or 1/(N_y N_x) for the 3-D factors without k_z,
+// whose z transforms are skipped (zfree)
+inline cx apply_mult(const Problem &P, int blk, const int *nk, cx X, double sc) {
   const int nd = P.ndim;
   auto nyq = [&](int a) { return nk[a] == P.Ns[a] / 2; };
   if (blk >= kImagBlk) {
@@ -467,19 +471,19 @@ inline cx apply_mult(const Problem &P, int blk, const int *nk, cx X) {
     if (blk <= nd) {
       int a = blk - 1;
       if (!nyq(a)) return {0.0, 0.0};
-      double val = P.kt[a][nk[a]] * P.invN;
+      double val = P.kt[a][nk[a]] * sc;
       return {X.re * val, X.im * val};
     }
     int qd = blk - 1 - nd;
     int a = qd / nd, b = qd % nd;
     if (a == b || nyq(a) == nyq(b)) return {0.0, 0.0};
-    double val = (P.kt[a][nk[a]] * P.kt[b][nk[b]]) * P.invN;
+    double val = (P.kt[a][nk[a]] * P.kt[b][nk[b]]) * sc;
     return {-(X.im * val), X.re * val};
   }
   if (blk <= nd) {
     int a = blk - 1;
     if (nyq(a)) return {0.0, 0.0};
-    double val = P.kt[a][nk[a]] * P.invN;
+    double val = P.kt[a][nk[a]] * sc;
     return {-(X.im * val), X.re * val};
   }
   int qd = blk - 1 - nd;
@@ -487,12 +491,27 @@ inline cx apply_mult(const Problem &P, int blk, const int *nk, cx X) {
   double val;
   if (a == b) {
     double k = P.kt[a][nk[a]];
-    val = (k * k) * P.invN;
+    val = (k * k) * sc;
   } else {
     if (nyq(a) != nyq(b)) return {0.0, 0.0};
-    val = (P.kt[a][nk[a]] * P.kt[b][nk[b]]) * P.invN;
+    val = (P.kt[a][nk[a]] * P.kt[b][nk[b]]) * sc;
   }
   return {-(X.re * val), -(X.im * val)};
+}
+
+
+// 3-D: a factor whose multiplier has no k_z (first and second derivatives in
+// y and x, and their imaginary-part codes).  The z transforms of such a
+// factor cancel, so it is multiplied in the y pass, right after the forward y
+// transform (scale 1/(N_y N_x)), and transformed back along y there; the z
+// pass and the inverse y pass carry only the other factors.
+inline bool zfree(const Problem &P, int blk) {
+  if (P.ndim != 3) return false;
+  if (blk >= kImagBlk) blk -= kImagBlk;
+  const int nd = P.ndim;
+  if (blk <= nd) return blk - 1 != 0;
+  const int q = blk - 1 - nd;
+  return q / nd != 0 && q % nd != 0;
 }
 
 // ---------------------------------------------------------------- RHS engine
@@ -552,7 +571,7 @@ struct Engine {
           const cx *s = &S[(size_t)fl[m].field * Nh];
           for (int k = 0; k < Nxh; ++k) {
             int nk[1] = {k};
-            tmp[k] = apply_mult(p, fl[m].blk, nk, s[k]);
+            tmp[k] = apply_mult(p, fl[m].blk, nk, s[k], p.invN);
           }
           std::memcpy(&F[(size_t)m * Nh], tmp.data(), sizeof(cx) * Nxh);
           rx.c2r(tmp.data(), &R[(size_t)m * N], scr.data());
@@ -563,11 +582,12 @@ struct Engine {
     const int Ny = p.Ns[nd - 2];
     const long planeY = (long)Ny * Nxh;  // one (z) slab
     const int Nz = nd == 3 ? p.Ns[0] : 1;
-    // 2. forward y for 3D (in place on S)
+    // 2. forward y for 3D (in place on S); the factors without k_z are
+    // multiplied here and transformed back along y at once
     if (nd == 3) {
 #pragma omp parallel
       {
-        std::vector<cx> a(Ny), b(Ny);
+        std::vector<cx> a(Ny), b(Ny), c(Ny);
 #pragma omp for schedule(static)
         for (long l = 0; l < (long)p.n * Nz * Nxh; ++l) {
           long f = l / ((long)Nz * Nxh), rem = l % ((long)Nz * Nxh);
@@ -576,6 +596,17 @@ struct Engine {
           for (int y = 0; y < Ny; ++y) a[y] = base[(long)y * Nxh];
           ay.run(a.data(), b.data(), a.data(), -1);
           for (int y = 0; y < Ny; ++y) base[(long)y * Nxh] = a[y];
+          for (int m = 0; m < nf; ++m) {
+            if (fl[m].field != f || !zfree(p, fl[m].blk)) continue;
+            int nk[3] = {0, 0, (int)kx};
+            for (int y = 0; y < Ny; ++y) {
+              nk[1] = y;
+              c[y] = apply_mult(p, fl[m].blk, nk, a[y], p.invNxy);
+            }
+            ay.run(c.data(), b.data(), c.data(), +1);
+            cx *dst = &F[(size_t)m * Nh + z * planeY + kx];
+            for (int y = 0; y < Ny; ++y) dst[(long)y * Nxh] = c[y];
+          }
         }
       }
     }
@@ -593,7 +624,7 @@ struct Engine {
           int f = (int)(l / nlines);
           long off = l % nlines;  // (y, kx) for 3D, kx for 2D
           bool any = false;
-          for (int m = 0; m < nf; ++m) any |= fl[m].field == f;
+          for (int m = 0; m < nf; ++m) any |= fl[m].field == f && !zfree(p, fl[m].blk);
           if (!any) continue;
           const cx *src = &S[(size_t)f * Nh + off];
           for (int i = 0; i < L0; ++i) a[i] = src[i * estride];
@@ -606,10 +637,10 @@ struct Engine {
             nk[1] = (int)off;
           }
           for (int m = 0; m < nf; ++m) {
-            if (fl[m].field != f) continue;
+            if (fl[m].field != f || zfree(p, fl[m].blk)) continue;
             for (int i = 0; i < L0; ++i) {
               nk[0] = i;
-              c[i] = apply_mult(p, fl[m].blk, nk, a[i]);
+              c[i] = apply_mult(p, fl[m].blk, nk, a[i], p.invN);
             }
             a0.run(c.data(), b.data(), c.data(), +1);
             cx *dst = &F[(size_t)m * Nh + off];
@@ -627,6 +658,7 @@ struct Engine {
         for (long l = 0; l < (long)nf * Nz * Nxh; ++l) {
           long m = l / ((long)Nz * Nxh), rem = l % ((long)Nz * Nxh);
           long z = rem / Nxh, kx = rem % Nxh;
+          if (zfree(p, fl[m].blk)) continue;  // already inverse-transformed along y
           cx *base = &F[(size_t)m * Nh + z * planeY + kx];
           for (int y = 0; y < Ny; ++y) a[y] = base[(long)y * Nxh];
           ay.run(a.data(), b.data(), a.data(), +1);

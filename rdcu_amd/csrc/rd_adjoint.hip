@@ -548,15 +548,21 @@ int loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const doub
   HIPCHECK(hipMemcpyAsync(fl7f, h7f.data(), h7f.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(hipMemcpyAsync(fl7s, h7s.data(), h7s.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
   const bool batched_spectra = c->ndim >= 2;  // in 1-D the row kernel reads S only
+  // 3-D: the repeated list's split (factors without k_z), kept by the handle
+  if (c->ndim == 3 && !c->zs7.zd.fstart &&
+      (st = build_zsplit(c, 7 * c->n, h7b, h7s, c->zs7)))
+    return st;
   auto enqueue_spectra7 = [&]() {
     const rd::Grid g0 = c->grid;
     const rd::Bufs b0 = c->bufs;
     const rd::FacList f0 = c->fl_used;
+    const ZSplit z0 = c->zs_used;
     const int n0 = c->n, nf0 = c->nf_used;
     c->grid.n = 7 * n0;
     c->n = 7 * n0;
     c->nf_used = nf7;
     c->fl_used = rd::FacList{nf7, fl7b, fl7f, fl7s};
+    c->zs_used = c->zs7;
     c->bufs.S = S7;
     c->bufs.F = F7;
     rd::StageCoef none{};
@@ -565,6 +571,7 @@ int loss_grad(rdcu_ctx *c, const double *y0, double t0, long long ns, const doub
     c->grid = g0;
     c->bufs = b0;
     c->fl_used = f0;
+    c->zs_used = z0;
     c->n = n0;
     c->nf_used = nf0;
   };

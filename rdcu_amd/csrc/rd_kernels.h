@@ -68,6 +68,7 @@ struct Grid {
   int nt_store;  // streaming stores in the x-forward and strided passes
   long N, rows, Nh;  // Nh = rows * Nxp
   double invN;
+  double invNxy;  // 3-D: 1/(N_y N_x), the scale of the factors without k_z (zfree_blk)
   const double *kt[3];  // wavenumbers per axis (device)
 };
 
@@ -94,12 +95,31 @@ struct DictP {
 
 // Transformed factors, grouped by field: factors of field f are
 // [fstart[f], fstart[f+1]); fblk[m] is the derivative block of factor m.
+// fslot (null: m itself) is factor m's slot in F, for a sublist of a
+// problem's factors.
 struct FacList {
   int nf;
   const int *fblk;
   const int *ffield;
   const int *fstart;
+  const int *fslot;
 };
+__host__ __device__ inline int fac_slot(const FacList &fl, int m) {
+  return fl.fslot ? fl.fslot[m] : m;
+}
+
+// 3-D: a factor whose multiplier has no k_z (first and second derivatives in
+// y and x, and their imaginary-part codes).  Its z transforms cancel, so it is
+// multiplied in the y pass, right after the forward y transform (scale
+// invNxy), and transformed back along y there; the z pass and the inverse y
+// pass carry only the other factors.
+__host__ __device__ inline bool zfree_blk(int blk, int nd) {
+  if (nd != 3) return false;
+  if (blk >= kImagBlk) blk -= kImagBlk;  // imaginary-part codes
+  if (blk <= nd) return blk - 1 != 0;
+  const int q = blk - 1 - nd;
+  return q / nd != 0 && q % nd != 0;
+}
 
 enum SrcKind { SRC_PTR = 0, SRC_Y = 1, SRC_STAGE = 2, SRC_YT = 3 };
 enum DstKind { DST_PTR = 0, DST_K1 = 1, DST_KS = 2, DST_K7 = 3 };
@@ -109,15 +129,17 @@ __device__ __forceinline__ bool attempt_active(const DevState *st) {
 }
 
 // Hermitian part of the reference's multiplier prod (i k_a) (rdcu.cu:78-99),
-// times 1/N, applied to X at wavenumber indices n0..n2 (last axis in
+// times the scale sc (1/N, or invNxy for the 3-D factors without k_z), applied
+// to X at wavenumber indices n0..n2 (last axis in
 // [0, Nx/2]).  Nyquist rules: d/dx_a vanishes on the N_a/2 plane; mixed
 // d2/dx_a dx_b vanishes where exactly one axis is at Nyquist.
-__device__ __forceinline__ cx apply_mult(const Grid &g, int blk, const int *nk, cx X) {
+__device__ __forceinline__ cx apply_mult(const Grid &g, int blk, const int *nk, cx X,
+                                         double sc) {
   const int nd = g.ndim;
   if (blk <= nd) {
     const int a = blk - 1;
     if (nk[a] == g.Ns[a] / 2) return {0.0, 0.0};
-    const double val = g.kt[a][nk[a]] * g.invN;
+    const double val = g.kt[a][nk[a]] * sc;
     return {-(X.im * val), X.re * val};
   }
   const int q = blk - 1 - nd;
@@ -125,11 +147,11 @@ __device__ __forceinline__ cx apply_mult(const Grid &g, int blk, const int *nk, 
   double val;
   if (a == b) {
     const double k = g.kt[a][nk[a]];
-    val = (k * k) * g.invN;
+    val = (k * k) * sc;
   } else {
     const bool na = nk[a] == g.Ns[a] / 2, nb = nk[b] == g.Ns[b] / 2;
     if (na != nb) return {0.0, 0.0};
-    val = (g.kt[a][nk[a]] * g.kt[b][nk[b]]) * g.invN;
+    val = (g.kt[a][nk[a]] * g.kt[b][nk[b]]) * sc;
   }
   return {-(X.re * val), -(X.im * val)};
 }
@@ -137,7 +159,8 @@ __device__ __forceinline__ cx apply_mult(const Grid &g, int blk, const int *nk, 
 // apply_mult, and for codes blk >= kImagBlk the imaginary part of Y (the
 // factor output of the LDS kernels; the integration path never sees these
 // codes and calls apply_mult, which keeps its registers).
-__device__ __forceinline__ cx apply_mult_y(const Grid &g, int blk, const int *nk, cx X) {
+__device__ __forceinline__ cx apply_mult_y(const Grid &g, int blk, const int *nk, cx X,
+                                           double sc) {
   const int nd = g.ndim;
   if (blk >= kImagBlk) {
     // -i times the anti-Hermitian part of the reference's multiplier m
@@ -149,7 +172,7 @@ __device__ __forceinline__ cx apply_mult_y(const Grid &g, int blk, const int *nk
     if (blk <= nd) {
       const int a = blk - 1;
       if (nk[a] != g.Ns[a] / 2) return {0.0, 0.0};
-      const double val = g.kt[a][nk[a]] * g.invN;
+      const double val = g.kt[a][nk[a]] * sc;
       return {X.re * val, X.im * val};
     }
     const int q = blk - 1 - nd;
@@ -157,10 +180,10 @@ __device__ __forceinline__ cx apply_mult_y(const Grid &g, int blk, const int *nk
     if (a == b) return {0.0, 0.0};
     const bool na = nk[a] == g.Ns[a] / 2, nb = nk[b] == g.Ns[b] / 2;
     if (na == nb) return {0.0, 0.0};
-    const double val = (g.kt[a][nk[a]] * g.kt[b][nk[b]]) * g.invN;
+    const double val = (g.kt[a][nk[a]] * g.kt[b][nk[b]]) * sc;
     return {-(X.im * val), X.re * val};
   }
-  return apply_mult(g, blk, nk, X);
+  return apply_mult(g, blk, nk, X, sc);
 }
 
 __device__ __forceinline__ double ipow(double v, int e) {
@@ -312,6 +335,11 @@ __global__ void k_row_r2c(Grid g, FftPlan px, const cx *__restrict__ twN, DevSta
 //   mode 1: forward FFT of S, then for every factor of the line's field the
 //           multiplier and inverse FFT -> F (axis 0)
 //   mode 2: inverse FFT in place on F      (3D axis 1)
+//   mode 3: mode 0, then for every factor of the line's field (the factors
+//           without k_z) the multiplier (scale invNxy) and inverse FFT -> F
+//           (3D axis 1)
+//   mode 4: the factor part of mode 3 on S already transformed (3D axis 1)
+// F positions are the factors' slots (fac_slot).
 // Tile id -> (group g, column tile ct); group g -> (g1, g2) = (g / gdiv, g % gdiv);
 // line base = g1*gs1 + g2*gs2 + ct*tc, element stride es.  For mode 1, g1 is the
 // field and the factor outputs sit at m*Nh + g2*gs2 + ct*tc.
@@ -345,7 +373,8 @@ __global__ void k_col(Grid g, FftPlan p, ColGeom cg, DevState *st, int check_act
   const int tile = blockIdx.x;
   const int grp = tile / cg.ntc, ct = tile - grp * cg.ntc;
   const int g1 = grp / cg.gdiv, g2 = grp - g1 * cg.gdiv;
-  if (mode == 1 && fl.fstart[g1] == fl.fstart[g1 + 1]) return;  // field with no factors
+  const bool fac = mode == 1 || mode == 3 || mode == 4;  // the multiply modes
+  if ((mode == 1 || mode == 4) && fl.fstart[g1] == fl.fstart[g1 + 1]) return;  // no factors
   const int c0 = ct * cg.tc;
   const int nc = (g.Nxh - c0) < cg.tc ? (g.Nxh - c0) : cg.tc;
   const int L = cg.L;
@@ -355,7 +384,7 @@ __global__ void k_col(Grid g, FftPlan p, ColGeom cg, DevState *st, int check_act
   const long lbase = (long)g2 * cg.gs2 + c0;
 
   cx *src;
-  if (mode == 2) src = b.F + (long)g1 * cg.gs1 + lbase;
+  if (mode == 2) src = b.F + (long)fac_slot(fl, g1) * cg.gs1 + lbase;
   else src = b.S + (long)g1 * cg.gs1 + lbase;
   for (int w = tid; w < tot; w += nt) {
     const int c = w % nc, e = w / nc;
@@ -371,27 +400,43 @@ __global__ void k_col(Grid g, FftPlan p, ColGeom cg, DevState *st, int check_act
     }
     return;
   }
-  cx *R = lds_fft<-1, true>(A, B, p, nc, 1, nc, tid, nt);
+  (void)fac;
+  cx *R = A;
+  if (mode != 4) R = lds_fft<-1, true>(A, B, p, nc, 1, nc, tid, nt);
+  if (mode == 3) {  // the forward spectra stay in S for the z pass
+    for (int w = tid; w < tot; w += nt) {
+      const int c = w % nc, e = w / nc;
+      src[e * cg.es + c] = R[e * nc + c];
+    }
+  }
   cx *Cb = (R == A) ? B : A;
   const int f = g1;
   const int m0 = fl.fstart[f], m1 = fl.fstart[f + 1];
+  // modes 3 and 4 run along y with the scale of the factors without k_z
+  const double msc = mode != 1 ? g.invNxy : g.invN;
   for (int m = m0; m < m1; ++m) {
     const int blk = fl.fblk[m];
     for (int w = tid; w < tot; w += nt) {
       const int c = w % nc, e = w / nc;
       int nk[3];
-      nk[0] = e;
-      if (g.ndim == 2) {
-        nk[1] = c0 + c;
-      } else {
-        nk[1] = g2;
+      if (mode != 1) {
+        nk[0] = 0;
+        nk[1] = e;
         nk[2] = c0 + c;
+      } else {
+        nk[0] = e;
+        if (g.ndim == 2) {
+          nk[1] = c0 + c;
+        } else {
+          nk[1] = g2;
+          nk[2] = c0 + c;
+        }
       }
-      Cb[e * nc + c] = apply_mult_y(g, blk, nk, R[e * nc + c]);
+      Cb[e * nc + c] = apply_mult_y(g, blk, nk, R[e * nc + c], msc);
     }
     __syncthreads();
     const cx *Q = lds_fft<+1, true>(Cb, C, p, nc, 1, nc, tid, nt);
-    cx *dst = b.F + (long)m * g.Nh + lbase;
+    cx *dst = b.F + (long)fac_slot(fl, m) * g.Nh + lbase;
     for (int w = tid; w < tot; w += nt) {
       const int c = w % nc, e = w / nc;
       dst[e * cg.es + c] = Q[e * nc + c];
@@ -444,8 +489,8 @@ __global__ void k_row_c2r(Grid g, FftPlan px, const cx *__restrict__ twN, DevSta
         const cx *row = b.S + (long)fl.ffield[m] * g.Nh;
         const int blk = fl.fblk[m];
         int nk0 = k, nk1 = L - k;
-        Av = apply_mult_y(g, blk, &nk0, row[k]);
-        Bv = apply_mult_y(g, blk, &nk1, row[L - k]);
+        Av = apply_mult_y(g, blk, &nk0, row[k], g.invN);
+        Bv = apply_mult_y(g, blk, &nk1, row[L - k], g.invN);
       } else {
         const cx *row = b.F + (long)m * g.Nh + r * g.Nxp;
         Av = row[k];

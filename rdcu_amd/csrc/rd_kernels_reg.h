@@ -142,23 +142,37 @@ struct Prefetch {
 // Multiplier value of derivative block blk at wavenumber indices nk with the
 // wavenumbers of each axis already loaded (kv) -- the same operations as
 // apply_mult.
-__device__ __forceinline__ cx mult_k(const Grid &g, int blk, const int *nk, const double *kv, cx X) {
+// The wavenumber of a strided line's element e: axis 0 (z, or y in 2-D), or
+// with ymul the 3-D y axis (constant indices: nk and kv stay in registers).
+__device__ __forceinline__ void line_k(bool ymul, int e, const double *ks, int *nk, double *kv) {
+  const double k = ks[e];
+  if (ymul) {
+    nk[1] = e;
+    kv[1] = k;
+  } else {
+    nk[0] = e;
+    kv[0] = k;
+  }
+}
+
+__device__ __forceinline__ cx mult_k(const Grid &g, int blk, const int *nk, const double *kv, cx X,
+                                     double sc) {
   const int nd = g.ndim;
   if (blk <= nd) {
     const int a = blk - 1;
     if (nk[a] == g.Ns[a] / 2) return {0.0, 0.0};
-    const double val = kv[a] * g.invN;
+    const double val = kv[a] * sc;
     return {-(X.im * val), X.re * val};
   }
   const int q = blk - 1 - nd;
   const int a = q / nd, b = q - a * nd;
   double val;
   if (a == b) {
-    val = (kv[a] * kv[a]) * g.invN;
+    val = (kv[a] * kv[a]) * sc;
   } else {
     const bool na = nk[a] == g.Ns[a] / 2, nb = nk[b] == g.Ns[b] / 2;
     if (na != nb) return {0.0, 0.0};
-    val = (kv[a] * kv[b]) * g.invN;
+    val = (kv[a] * kv[b]) * sc;
   }
   return {-(X.re * val), -(X.im * val)};
 }
@@ -286,7 +300,15 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     m0 = fl.fstart[g1] + part * fpw;
     m1 = fl.fstart[g1 + 1];
     if (m1 > m0 + fpw) m1 = m0 + fpw;
+  } else if (mode == 3) {  // every factor of the field (those without k_z)
+    m0 = fl.fstart[g1];
+    m1 = fl.fstart[g1 + 1];
   }
+  // the line axis of the multiply: z (axis 0) in mode 1, y in mode 3, whose
+  // factors take the scale without the z round trip
+  const bool ymul = !FO && mode == 3;
+  const double msc = ymul ? g.invNxy : g.invN;
+  const double *ktl = ymul ? g.kt[1] : g.kt[0];
   const bool idle = mode == 1 && m0 >= m1;
   const int L = LS > 0 ? LS : cg.L, T = L / E, tc = cg.tc;
   // tiles of tc columns from column 0 (whole 16-byte-column groups per
@@ -307,9 +329,9 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   // factors; the wavenumbers of axis 0 follow it
   const bool keep = m1 - m0 > 1;
   double *k0l = reinterpret_cast<double *>(buf + tc * padded_len(L) + (keep ? tc * L : 0));
-  const double *k0s = TG ? g.kt[0] : k0l;
+  const double *k0s = TG ? ktl : k0l;
   const long lbase = (long)g2 * cg.gs2 + c0 + c;
-  cx *src = (mode == 2 ? b.F : b.S) + (long)g1 * cg.gs1 + lbase;
+  cx *src = (mode == 2 ? b.F + (long)fac_slot(fl, g1) * cg.gs1 : b.S + (long)g1 * cg.gs1) + lbase;
   cx v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = act ? src[(long)(t + T * q) * cg.es] : cx{0.0, 0.0};
@@ -331,20 +353,26 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
       kv[2] = g.kt[2][kx];
     }
     blk0 = idle ? 0 : fl.fblk[m0];
+  } else if (ymul) {
+    nk[0] = 0;
+    kv[0] = 0.0;
+    nk[2] = kx;
+    kv[2] = g.kt[2][kx];
+    blk0 = m0 < m1 ? fl.fblk[m0] : 0;
   }
   Prefetch<4, cx> pf_tw;
   Prefetch<4, double> pf_k;
   if constexpr (!TG) {
     pf_tw.load(p.tw, L, tid, blockDim.x);
-    if (mode == 1) pf_k.load(g.kt[0], L, tid, blockDim.x);
+    if (mode == 1 || ymul) pf_k.load(ktl, L, tid, blockDim.x);
   }
   if (idle || (chk && !attempt_active(chk))) return;
   if constexpr (!TG) {
     pf_tw.store(twl, p.tw, L, tid, blockDim.x);
-    if (mode == 1) pf_k.store(k0l, g.kt[0], L, tid, blockDim.x);
+    if (mode == 1 || ymul) pf_k.store(k0l, ktl, L, tid, blockDim.x);
   }
   cx *lb = buf + c;
-  if (FO || mode != 1) {
+  if (FO || (mode != 1 && mode != 3)) {
     if (mode == 0) fft_any<E, -1, LS>(v, lb, tc, tw, p, t, T);
     else fft_any<E, +1, LS>(v, lb, tc, tw, p, t, T);
     if (act) {
@@ -354,17 +382,23 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     return;
   }
   fft_any<E, -1, LS>(v, lb, tc, tw, p, t, T);
+  if (ymul) {  // the forward spectra stay in S for the z pass
+    if (act) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) st_g<NT>(src + (long)(t + T * q) * cg.es, v[q]);
+    }
+    if (m0 >= m1) return;
+  }
   if (!keep) {  // one factor: the spectrum goes from registers to its multiply
     cx w[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      nk[0] = t + T * q;
-      kv[0] = k0s[t + T * q];
-      w[q] = mult_k(g, blk0, nk, kv, v[q]);
+      line_k(ymul, t + T * q, k0s, nk, kv);
+      w[q] = mult_k(g, blk0, nk, kv, v[q], msc);
     }
     fft_any<E, +1, LS>(w, lb, tc, tw, p, t, T);
     if (act) {
-      cx *dst = b.F + (long)m0 * g.Nh + lbase;
+      cx *dst = b.F + (long)fac_slot(fl, m0) * g.Nh + lbase;
 #pragma unroll
       for (int q = 0; q < E; ++q) st_g<NT>(dst + (long)(t + T * q) * cg.es, w[q]);
     }
@@ -380,13 +414,12 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     cx w[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      nk[0] = t + T * q;
-      kv[0] = k0s[t + T * q];
-      w[q] = mult_k(g, blk, nk, kv, xs[(t + T * q) * tc + c]);
+      line_k(ymul, t + T * q, k0s, nk, kv);
+      w[q] = mult_k(g, blk, nk, kv, xs[(t + T * q) * tc + c], msc);
     }
     fft_any<E, +1, LS>(w, lb, tc, tw, p, t, T);
     if (act) {
-      cx *dst = b.F + (long)m * g.Nh + lbase;
+      cx *dst = b.F + (long)fac_slot(fl, m) * g.Nh + lbase;
 #pragma unroll
       for (int q = 0; q < E; ++q) st_g<NT>(dst + (long)(t + T * q) * cg.es, w[q]);
     }
@@ -444,8 +477,8 @@ __device__ __forceinline__ void c2r_pre(const Grid &g, const FacList &fl, const 
     cx Av = row[k], Bv = row[L - k];
     if (d1) {
       int nk0 = k, nk1 = L - k;
-      Av = apply_mult(g, blk, &nk0, Av);
-      Bv = apply_mult(g, blk, &nk1, Bv);
+      Av = apply_mult(g, blk, &nk0, Av, g.invN);
+      Bv = apply_mult(g, blk, &nk1, Bv, g.invN);
     }
     Bv = cconj(Bv);
     if (k == 0) {

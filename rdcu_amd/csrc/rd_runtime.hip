@@ -144,6 +144,14 @@ int scratch_base(int dev, char **out) {
 // Calls on one GPU share its array: they are serialised within the process.
 std::mutex &scratch_mutex(int dev) { return g_scratch_call_mu[dev]; }
 
+// 3-D: a factor list split into the factors without k_z (zf: multiplied in
+// the y pass) and the others (zd: through the z pass and the inverse y pass),
+// each with the slots of its factors in the full list.
+struct ZSplit {
+  rd::FacList zf{}, zd{};
+  int zf_max = 0, zd_max = 0;  // most factors of one field
+};
+
 }  // namespace
 
 struct rdcu_ctx {
@@ -181,6 +189,9 @@ struct rdcu_ctx {
   int col_lds = 0;
   rd::FacList fl_used{}, fl_all{};
   rd::FacList fl_imag{};  // fl_all with the imaginary-part codes (Y output)
+  // 3-D splits of fl_used, fl_all and fl_imag, and of the adjoint's list for
+  // seven stage inputs side by side (zs7: built at its first use)
+  ZSplit zs_used, zs_all, zs_imag, zs7;
   rd::DevState *st = nullptr;
   rd::DevState *sh = nullptr;  // pinned host shadow
   double *rhs_in = nullptr, *rhs_out = nullptr, *Yreal = nullptr, *yeval = nullptr;
@@ -273,6 +284,37 @@ struct rdcu_ctx {
 
 namespace {
 
+// Splits the factor list (blocks blk, field f's factors [start[f],
+// start[f+1])) into zf (the factors without k_z) and zd (the others) and
+// uploads both halves.
+int build_zsplit(rdcu_ctx *c, int nfields, const std::vector<int> &blk,
+                 const std::vector<int> &start, ZSplit &z) {
+  std::vector<int> b[2], f[2], sl[2], st[2];
+  for (int k = 0; k < 2; ++k) st[k].assign(nfields + 1, 0);
+  for (int fd = 0; fd < nfields; ++fd) {
+    for (int k = 0; k < 2; ++k) st[k][fd] = (int)b[k].size();
+    for (int m = start[fd]; m < start[fd + 1]; ++m) {
+      const int k = rd::zfree_blk(blk[m], c->ndim) ? 0 : 1;
+      b[k].push_back(blk[m]);
+      f[k].push_back(fd);
+      sl[k].push_back(m);
+    }
+  }
+  rd::FacList *out[2] = {&z.zf, &z.zd};
+  int *mx[2] = {&z.zf_max, &z.zd_max};
+  for (int k = 0; k < 2; ++k) {
+    st[k][nfields] = (int)b[k].size();
+    int *db = nullptr, *df = nullptr, *ds = nullptr, *dst = nullptr, s;
+    if ((s = c->upload(&db, b[k])) || (s = c->upload(&df, f[k])) || (s = c->upload(&ds, sl[k])) ||
+        (s = c->upload(&dst, st[k])))
+      return s;
+    *out[k] = rd::FacList{(int)b[k].size(), db, df, dst, ds};
+    *mx[k] = 0;
+    for (int fd = 0; fd < nfields; ++fd) *mx[k] = std::max(*mx[k], st[k][fd + 1] - st[k][fd]);
+  }
+  return RDCU_OK;
+}
+
 // Kernel launches of one RHS evaluation, in order.  stage 2..7: input =
 // DP45 stage sum, output = stage slot; stage 0: input/output given by
 // kind/pointer.
@@ -300,16 +342,21 @@ std::vector<Launch> rhs_launches(rdcu_ctx *c, int stage, int src_kind, const dou
                }});
   const int nd = c->ndim;
   const int ntc = (c->Nxh + c->col_tc - 1) / c->col_tc;
+  // 3-D: the factors without k_z in the y forward pass (mode 3), the others
+  // through the z pass and the inverse y pass
+  const ZSplit &zs = c->zs_used;
   if (nd == 3) {
     rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc,
                    c->col_tc, 1};
     dim3 gg((unsigned)((long)c->n * c->Ns[0] * ntc));
+    const bool zf = zs.zf.nf > 0;
     v.push_back({"col_fwd", [=]() {
                    hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream,
-                                      c->grid, c->p1, cg, c->st, check, c->bufs, c->fl_used, 0);
+                                      c->grid, c->p1, cg, c->st, check, c->bufs,
+                                      zf ? zs.zf : c->fl_used, zf ? 3 : 0);
                  }});
   }
-  if (nd >= 2) {
+  if (nd == 2 || (nd == 3 && zs.zd.nf > 0)) {
     rd::ColGeom cg{};
     cg.L = c->Ns[0];
     cg.ntc = ntc;
@@ -326,18 +373,19 @@ std::vector<Launch> rhs_launches(rdcu_ctx *c, int stage, int src_kind, const dou
       cg.gs2 = c->Nxp;
     }
     dim3 gg((unsigned)((long)c->n * cg.gdiv * ntc));
+    const rd::FacList fl = nd == 3 ? zs.zd : c->fl_used;
     v.push_back({"col_mult", [=]() {
                    hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream,
-                                      c->grid, c->p0, cg, c->st, check, c->bufs, c->fl_used, 1);
+                                      c->grid, c->p0, cg, c->st, check, c->bufs, fl, 1);
                  }});
   }
-  if (nd == 3 && c->nf_used > 0) {
+  if (nd == 3 && zs.zd.nf > 0) {
     rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc,
                    c->col_tc, 1};
-    dim3 gg((unsigned)((long)c->nf_used * c->Ns[0] * ntc));
+    dim3 gg((unsigned)((long)zs.zd.nf * c->Ns[0] * ntc));
     v.push_back({"col_inv", [=]() {
                    hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream,
-                                      c->grid, c->p1, cg, c->st, check, c->bufs, c->fl_used, 2);
+                                      c->grid, c->p1, cg, c->st, check, c->bufs, zs.zd, 2);
                  }});
   }
   rd::ErrCoef ec{};
@@ -493,7 +541,10 @@ void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups
   if (cg.nparts < 1) cg.nparts = 1;
   cg.nblk = (int)(ngroups * cg.ntc * cg.nparts);
   dim3 g((unsigned)((cg.nblk + 7) / 8 * 8));
-  const size_t lds = col_lds(cg.L, cg.tc, E, mode == 1, fpw);
+  // mode 3 (3-D y pass with the factors without k_z): all of a field's such
+  // factors in one block
+  const size_t lds = mode == 3 ? col_lds(cg.L, cg.tc, E, true, std::max(1, c->zs_used.zf_max))
+                               : col_lds(cg.L, cg.tc, E, mode == 1, fpw);
   if (c->grid.nt_store)
     go_col_l<E, true>(c, g, dim3(cg.tc * T), lds, p, cg, check, fl, mode, fpw);
   else
@@ -607,13 +658,19 @@ std::vector<Launch> col_launches_reg(rdcu_ctx *c, int check, int tc, int ntc) {
   }
   const int nd = c->ndim;
   const int bal = col_tiles(c, tc, ntc);
+  // 3-D: the y forward pass also makes the factors without k_z (mode 3);
+  // the z pass and the inverse y pass take the others
+  const ZSplit &zs = c->zs_used;
   if (nd == 3) {
     rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc, tc, 1};
     cg.bal = bal;
     long ng = (long)c->n * c->Ns[0];
-    v.push_back({"col_fwd", [=]() { colr(c, c->p1, c->E1, cg, ng, check, c->fl_used, 0); }});
+    if (zs.zf.nf > 0)
+      v.push_back({"col_fwd", [=]() { colr(c, c->p1, c->E1, cg, ng, check, zs.zf, 3); }});
+    else
+      v.push_back({"col_fwd", [=]() { colr(c, c->p1, c->E1, cg, ng, check, c->fl_used, 0); }});
   }
-  if (nd >= 2) {
+  if (nd == 2 || (nd == 3 && zs.zd.nf > 0)) {
     rd::ColGeom cg{};
     cg.L = c->Ns[0];
     cg.ntc = ntc;
@@ -629,13 +686,14 @@ std::vector<Launch> col_launches_reg(rdcu_ctx *c, int check, int tc, int ntc) {
       cg.gs2 = c->Nxp;
     }
     long ng = (long)c->n * cg.gdiv;
-    v.push_back({"col_mult", [=]() { colr(c, c->p0, c->E0, cg, ng, check, c->fl_used, 1); }});
+    const rd::FacList fl = nd == 3 ? zs.zd : c->fl_used;
+    v.push_back({"col_mult", [=]() { colr(c, c->p0, c->E0, cg, ng, check, fl, 1); }});
   }
-  if (nd == 3 && c->nf_used > 0) {
+  if (nd == 3 && zs.zd.nf > 0) {
     rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc, tc, 1};
     cg.bal = bal;
-    long ng = (long)c->nf_used * c->Ns[0];
-    v.push_back({"col_inv", [=]() { colr(c, c->p1, c->E1, cg, ng, check, c->fl_used, 2); }});
+    long ng = (long)zs.zd.nf * c->Ns[0];
+    v.push_back({"col_inv", [=]() { colr(c, c->p1, c->E1, cg, ng, check, zs.zd, 2); }});
   }
   return v;
 }
@@ -919,7 +977,10 @@ void build_jit(rdcu_ctx *c) {
 // of a CU (axis 0 keeps the forward spectrum; in 3-D axis 1 does not).
 bool col_fits(const rdcu_ctx *c, int tc, int fpw) {
   size_t lds = col_lds(c->Ns[0], tc, c->E0, true, fpw);
-  if (c->ndim == 3) lds = std::max(lds, col_lds(c->Ns[1], tc, c->E1, false, 1));
+  if (c->ndim == 3) {
+    const int zf = c->zs_used.zf_max;
+    lds = std::max(lds, col_lds(c->Ns[1], tc, c->E1, zf > 0, std::max(1, zf)));
+  }
   return lds <= 160 * 1024;
 }
 
@@ -1173,13 +1234,21 @@ int once_per_device(int dev, int slot, int (*fn)()) {
 }
 
 // Every factor of fl (every derivative block of every field: fl_all, or
-// their imaginary parts: fl_imag), real space, into out (S must hold the
-// current forward spectra; it is only read).
-int enqueue_all_factors(rdcu_ctx *c, const rd::FacList &fl, double *out) {
+// their imaginary parts: fl_imag; zs its 3-D split), real space, into out (S
+// must hold the current forward spectra, in 3-D after the y forward pass; it
+// is only read).
+int enqueue_all_factors(rdcu_ctx *c, const rd::FacList &fl, const ZSplit &zs, double *out) {
   if (c->nf_all == 0) return RDCU_OK;
   const int nd = c->ndim;
   const int ntc = (c->Nxh + c->col_tc - 1) / c->col_tc;
-  if (nd >= 2) {
+  if (nd == 3 && zs.zf.nf > 0) {  // the factors without k_z: multiply, inverse y
+    rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc,
+                   c->col_tc, 1};
+    dim3 gg((unsigned)((long)c->n * c->Ns[0] * ntc));
+    hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream, c->grid, c->p1, cg,
+                       c->st, 0, c->bufs, zs.zf, 4);
+  }
+  if (nd == 2 || (nd == 3 && zs.zd.nf > 0)) {
     rd::ColGeom cg{};
     cg.L = c->Ns[0];
     cg.ntc = ntc;
@@ -1195,14 +1264,14 @@ int enqueue_all_factors(rdcu_ctx *c, const rd::FacList &fl, double *out) {
     }
     dim3 gg((unsigned)((long)c->n * cg.gdiv * ntc));
     hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream, c->grid, c->p0, cg,
-                       c->st, 0, c->bufs, fl, 1);
+                       c->st, 0, c->bufs, nd == 3 ? zs.zd : fl, 1);
   }
-  if (nd == 3) {
+  if (nd == 3 && zs.zd.nf > 0) {
     rd::ColGeom cg{c->Ns[1], (long)c->Nxp, c->Ns[0], c->Nh, (long)c->Ns[1] * c->Nxp, ntc,
                    c->col_tc, 1};
-    dim3 gg((unsigned)((long)c->nf_all * c->Ns[0] * ntc));
+    dim3 gg((unsigned)((long)zs.zd.nf * c->Ns[0] * ntc));
     hipLaunchKernelGGL(rd::k_col, gg, dim3(c->col_thr), c->lds_col, c->stream, c->grid, c->p1, cg,
-                       c->st, 0, c->bufs, fl, 2);
+                       c->st, 0, c->bufs, zs.zd, 2);
   }
   rd::ErrCoef ec{};
   hipLaunchKernelGGL(rd::k_row_c2r, dim3((unsigned)c->rows), dim3(c->c2r_thr), c->lds_c2r_all,
@@ -1490,6 +1559,7 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
   g.rows = c->rows;
   g.Nh = c->Nh;
   g.invN = 1.0 / (double)c->N;
+  g.invNxy = c->ndim == 3 ? 1.0 / (double)((long)c->Ns[1] * c->Ns[2]) : g.invN;
   for (int a = 0; a < c->ndim; ++a) {
     const int Na = c->Ns[a];
     std::vector<double> kt(Na);
@@ -1555,6 +1625,15 @@ int create_ctx(const rdcu_problem *p, double atol, double rtol, int fixedstep, c
     int *ib;
     if ((st = c->upload(&ib, iblk))) return st;
     c->fl_imag = rd::FacList{c->nf_all, ib, af, as};
+    if (c->ndim == 3) {
+      if ((st = build_zsplit(c.get(), c->n, ublk, ustart, c->zs_used)) ||
+          (st = build_zsplit(c.get(), c->n, ablk, astart, c->zs_all)) ||
+          (st = build_zsplit(c.get(), c->n, iblk, astart, c->zs_imag)))
+        return st;
+      // the z pass splits only the factors that go through it
+      c->max_fpf = std::max(1, c->zs_used.zd_max);
+      c->cr_fpw = c->max_fpf;
+    }
   }
   if ((st = choose_shapes(c.get()))) return st;
   // register-resident path when every transform length allows it
@@ -1865,12 +1944,12 @@ int enqueue_rhs_out(rdcu_ctx *c, const double *y_dev, double *f_out, double *Y_o
     const size_t ny = 2 * (size_t)c->nblk * nN;
     if (!c->Ycx && (st = c->dalloc(&c->Ycx, ny))) return st;
     const dim3 pg((unsigned)((N + 255) / 256), (unsigned)(c->nblk * c->n));
-    if ((st = enqueue_all_factors(c, c->fl_all, c->Yreal))) return st;
+    if ((st = enqueue_all_factors(c, c->fl_all, c->zs_all, c->Yreal))) return st;
     hipLaunchKernelGGL(rd::k_pack_y, pg, dim3(256), 0, c->stream, y_dev, c->Yreal,
                        (double2 *)c->Ycx, c->n, c->nblk, N);
     // imaginary parts (Nyquist leakage), through the same buffer
     if (c->nf_all > 0) {
-      if ((st = enqueue_all_factors(c, c->fl_imag, c->Yreal))) return st;
+      if ((st = enqueue_all_factors(c, c->fl_imag, c->zs_imag, c->Yreal))) return st;
       hipLaunchKernelGGL(rd::k_pack_y_imag, pg, dim3(256), 0, c->stream, c->Yreal,
                          (double2 *)c->Ycx, c->n, c->nblk, c->ndim, N);
     }

@@ -10,14 +10,18 @@ import bench  # noqa: E402
 
 
 def test_algorithmic_bytes_match_the_closed_form():
-    """B_rhs = 8N[n(2 ndim + 1) + 2M(ndim - 1)]: 37,748,736 B at 512^2 (n=2, M=4)
-    and 5.10 GB at 256^3 (n=2, M=6)."""
+    """B_rhs = 8N[n(2 ndim + 1) + 2M] in 2-D: 37,748,736 B at 512^2 (n=2, M=4).
+    In 3-D the factors without k_z skip the z pass and the inverse y pass:
+    8N[7n + 2M + 2Mz], 4.03 GB at 256^3 (n=2, M=6, Mz=2), against 5.10 GB
+    when every factor goes through both (Mz = M)."""
     kb = bench.algorithmic_bytes(2, 4, [512, 512])
     assert kb["row_fwd"] + kb["col_mult"] + kb["row_inv"] == 37748736
     N = 256 ** 3
+    keys = ("row_fwd", "col_fwd", "col_mult", "col_inv", "row_inv")
     kb3 = bench.algorithmic_bytes(2, 6, [256] * 3)
-    total = sum(kb3[k] for k in ("row_fwd", "col_fwd", "col_mult", "col_inv", "row_inv"))
-    assert total == 8 * N * (2 * 7 + 2 * 6 * 2)
+    assert sum(kb3[k] for k in keys) == 8 * N * (7 * 2 + 2 * 6 + 2 * 2)
+    kb3_all = bench.algorithmic_bytes(2, 6, [256] * 3, Mz=6)
+    assert sum(kb3_all[k] for k in keys) == 8 * N * (2 * 7 + 2 * 6 * 2)
 
 
 def test_measured_traffic_comes_from_the_counter_table():
@@ -54,8 +58,10 @@ def test_attempt_bytes_match_the_per_kernel_listing():
             total += 6 * mb if "," in cells[0] else mb
     assert abs(total - 339.7) < 0.2
     assert abs(bench.attempt_bytes(2, 4, [512, 512]) / 1e6 - total) < 0.2
-    # 3-D adds the y forward (2A) and y inverse (2F) passes to each of the six RHS
+    # 3-D adds to each of the six RHS the y forward pass (2A, and the writes of
+    # the factors without k_z moved out of the last-axis pass) and the inverse
+    # y pass of the factors with k_z (2Fz)
     N = 256 ** 3
-    A, F = 16 * N, 48 * N
+    A, Fz = 16 * N, 16 * N
     assert bench.attempt_bytes(2, 6, [256] * 3) - bench.attempt_bytes(2, 6, [256, 256 * 256]) \
-        == 6 * (2 * A + 2 * F)
+        == 6 * (2 * A + 2 * Fz)

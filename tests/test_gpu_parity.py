@@ -304,15 +304,15 @@ def test_sixteen_values_per_thread_columns_bitwise(hlib, olib, monkeypatch, Ns, 
 @pytest.mark.parametrize("Ns,Ls,shapes", [([512, 512], [200.0, 200.0], "8,4,2,4,2,8"),
                                           ([512, 512], [200.0, 200.0], "8,4,1,4,2,8"),
                                           ([256, 256], [100.0, 100.0], "4,4,2,4,2,16"),
-                                          ([256, 16, 32], [60.0, 20.0, 30.0], "4,4,2,4,1,8"),
+                                          ([256, 16, 32], [60.0, 20.0, 30.0], "4,4,1,4,1,8"),
                                           ([512, 512], [200.0, 200.0], "2,5,1,4,1,8,64"),
                                           ([512, 512], [200.0, 200.0], "2,3,2,4,1,8,128"),
                                           ([96, 48, 64], [40.0, 20.0, 30.0], "4,2,1,4,1,8,11"),
                                           ([512, 512], [200.0, 200.0], "2,4,1,4,1,8,0,1"),
-                                          ([32, 64, 256], [30.0, 40.0, 90.0], "4,4,2,4,1,8,0,1"),
+                                          ([32, 64, 256], [30.0, 40.0, 90.0], "4,4,1,4,1,8,0,1"),
                                           ([512, 512], [200.0, 200.0], "2,4,1,4,2,8,0,2"),
-                                          ([32, 64, 256], [30.0, 40.0, 90.0], "4,4,2,4,1,8,0,2"),
-                                          ([32, 64, 256], [30.0, 40.0, 90.0], "4,4,2,4,1,8,0,3"),
+                                          ([32, 64, 256], [30.0, 40.0, 90.0], "4,4,1,4,1,8,0,2"),
+                                          ([32, 64, 256], [30.0, 40.0, 90.0], "4,4,1,4,1,8,0,3"),
                                           ([96, 192], [40.0, 80.0], "4,4,1,4,1,8,0,3")],
                          ids=["2d-512-fpw2", "2d-512-fpw1", "2d-256-e16", "3d-256z",
                               "2d-512-64tiles", "2d-512-128tiles-fpw2", "3d-11tiles",
