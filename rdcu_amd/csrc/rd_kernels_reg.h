@@ -139,9 +139,6 @@ struct Prefetch {
   }
 };
 
-// Multiplier value of derivative block blk at wavenumber indices nk with the
-// wavenumbers of each axis already loaded (kv) -- the same operations as
-// apply_mult.
 // The wavenumber of a strided line's element e: axis 0 (z, or y in 2-D), or
 // with ymul the 3-D y axis (constant indices: nk and kv stay in registers).
 __device__ __forceinline__ void line_k(bool ymul, int e, const double *ks, int *nk, double *kv) {
@@ -155,6 +152,9 @@ __device__ __forceinline__ void line_k(bool ymul, int e, const double *ks, int *
   }
 }
 
+// Multiplier value of derivative block blk at wavenumber indices nk with the
+// wavenumbers of each axis already loaded (kv), scaled by sc -- the same
+// operations as apply_mult.
 __device__ __forceinline__ cx mult_k(const Grid &g, int blk, const int *nk, const double *kv, cx X,
                                      double sc) {
   const int nd = g.ndim;
