@@ -279,7 +279,10 @@ __global__ __launch_bounds__(256) void k_rowf_reg(Grid g, FftPlan px, const cx *
 // FO: built for modes 0 and 2 only (the transforms of the 3-D y passes): the
 // multiply path and its registers are compiled out, so that more blocks fit a
 // CU (the runtime launches it for the y forward pass).
-template <int E, bool NT = false, int LS = 0, bool TG = false, bool FO = false>
+// KR: a block with several factors keeps the forward spectrum in registers
+// instead of LDS (built for the 3-D y pass with the factors without k_z, 8
+// values per thread: less LDS per block, more blocks per CU).
+template <int E, bool NT = false, int LS = 0, bool TG = false, bool FO = false, bool KR = false>
 __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan &p,
                                          const ColGeom &cg, const Bufs &b, const FacList &fl,
                                          int mode_arg, int fpw, const DevState *chk = nullptr) {
@@ -327,7 +330,7 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
   const cx *tw = TG ? p.tw : twl;
   // mode 1: the forward spectrum is kept in LDS when the block has several
   // factors; the wavenumbers of axis 0 follow it
-  const bool keep = m1 - m0 > 1;
+  const bool keep = !KR && m1 - m0 > 1;
   double *k0l = reinterpret_cast<double *>(buf + tc * padded_len(L) + (keep ? tc * L : 0));
   const double *k0s = TG ? ktl : k0l;
   const long lbase = (long)g2 * cg.gs2 + c0 + c;
@@ -389,6 +392,24 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
     }
     if (m0 >= m1) return;
   }
+  if constexpr (KR) {  // each factor's multiply from the spectrum in registers
+    for (int m = m0; m < m1; ++m) {
+      const int blk = m == m0 ? blk0 : fl.fblk[m];
+      cx w[E];
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        line_k(ymul, t + T * q, k0s, nk, kv);
+        w[q] = mult_k(g, blk, nk, kv, v[q], msc);
+      }
+      fft_any<E, +1, LS>(w, lb, tc, tw, p, t, T);
+      if (act) {
+        cx *dst = b.F + (long)fac_slot(fl, m) * g.Nh + lbase;
+#pragma unroll
+        for (int q = 0; q < E; ++q) st_g<NT>(dst + (long)(t + T * q) * cg.es, w[q]);
+      }
+    }
+    return;
+  }
   if (!keep) {  // one factor: the spectrum goes from registers to its multiply
     cx w[E];
 #pragma unroll
@@ -433,13 +454,13 @@ __device__ __forceinline__ void col_body(int lblk, const Grid &g, const FftPlan 
 template <int E, int LS = 0>
 constexpr int col_max_threads() { return E == 8 && (LS == 0 || (LS > 0 && LS % 3 != 0)) ? 512 : 256; }
 
-template <int E, bool NT, int LS = 0, bool TG = false, bool FO = false>
+template <int E, bool NT, int LS = 0, bool TG = false, bool FO = false, bool KR = false>
 __global__ __launch_bounds__((col_max_threads<E, LS>())) void k_col_reg(Grid g, FftPlan p, ColGeom cg, DevState *st,
                                                  int check_active, Bufs b, FacList fl, int mode,
                                                  int fpw) {
   const int lblk = xcd_block(blockIdx.x, gridDim.x);
   if (lblk >= cg.nblk) return;
-  col_body<E, NT, LS, TG, FO>(lblk, g, p, cg, b, fl, mode, fpw, check_active ? st : nullptr);
+  col_body<E, NT, LS, TG, FO, KR>(lblk, g, p, cg, b, fl, mode, fpw, check_active ? st : nullptr);
 }
 
 // Inverse x pass for one row position r, all fields: factor lines in groups

@@ -492,6 +492,13 @@ void go_col(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, const rd::FftPlan &p, const
                          cg, c->st, check, c->bufs, fl, mode, fpw);
       return;
     }
+    // the 3-D y pass with the factors without k_z: their forward spectrum
+    // stays in registers (col_kr)
+    if (mode == 3) {
+      hipLaunchKernelGGL((rd::k_col_reg<E, NT, LS, false, false, true>), g, b, lds, c->stream,
+                         c->grid, p, cg, c->st, check, c->bufs, fl, mode, fpw);
+      return;
+    }
   }
   hipLaunchKernelGGL((rd::k_col_reg<E, NT, LS>), g, b, lds, c->stream, c->grid, p, cg, c->st,
                      check, c->bufs, fl, mode, fpw);
@@ -506,6 +513,10 @@ bool col_tg(int L, int E) { return E == 8 && L >= kColTgMin && !odd_len(L); }
 // LDS of a strided-pass block of tc columns of length L (mode 1: with the
 // wavenumbers, and with the kept spectrum when a block takes several
 // factors, fpw > 1).
+// The strided kernel keeps a multi-factor forward spectrum in registers
+// (k_col_reg<..., KR>) for the 3-D y pass (mode 3) with 8 values per thread.
+bool col_kr(int L, int E) { return E == 8 && !col_tg(L, E); }
+
 size_t col_lds(int L, int tc, int E, bool mode1, int fpw) {
   const bool tg = col_tg(L, E), keep = mode1 && fpw > 1;
   return (size_t)((tg ? 0 : L) + tc * rd::padded_len(L) + (keep ? tc * L : 0)) * sizeof(cx) +
@@ -543,8 +554,9 @@ void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups
   dim3 g((unsigned)((cg.nblk + 7) / 8 * 8));
   // mode 3 (3-D y pass with the factors without k_z): all of a field's such
   // factors in one block
-  const size_t lds = mode == 3 ? col_lds(cg.L, cg.tc, E, true, std::max(1, c->zs_used.zf_max))
-                               : col_lds(cg.L, cg.tc, E, mode == 1, fpw);
+  const size_t lds =
+      mode == 3 ? col_lds(cg.L, cg.tc, E, true, col_kr(cg.L, E) ? 1 : std::max(1, c->zs_used.zf_max))
+                : col_lds(cg.L, cg.tc, E, mode == 1, fpw);
   if (c->grid.nt_store)
     go_col_l<E, true>(c, g, dim3(cg.tc * T), lds, p, cg, check, fl, mode, fpw);
   else
@@ -979,7 +991,8 @@ bool col_fits(const rdcu_ctx *c, int tc, int fpw) {
   size_t lds = col_lds(c->Ns[0], tc, c->E0, true, fpw);
   if (c->ndim == 3) {
     const int zf = c->zs_used.zf_max;
-    lds = std::max(lds, col_lds(c->Ns[1], tc, c->E1, zf > 0, std::max(1, zf)));
+    lds = std::max(lds, col_lds(c->Ns[1], tc, c->E1, zf > 0,
+                                col_kr(c->Ns[1], c->E1) ? 1 : std::max(1, zf)));
   }
   return lds <= 160 * 1024;
 }
@@ -1211,7 +1224,19 @@ int allow_big_lds() {
       (const void *)rd::k_col_reg<8, false, kColLs3[1], false, true>,
       (const void *)rd::k_col_reg<8, true, kColLs3[1], false, true>,
       (const void *)rd::k_col_reg<8, false, 0, true>,
-      (const void *)rd::k_col_reg<8, true, 0, true>};
+      (const void *)rd::k_col_reg<8, true, 0, true>,
+      (const void *)rd::k_col_reg<8, false, 0, false, false, true>,
+      (const void *)rd::k_col_reg<8, true, 0, false, false, true>,
+      (const void *)rd::k_col_reg<8, false, -1, false, false, true>,
+      (const void *)rd::k_col_reg<8, true, -1, false, false, true>,
+      (const void *)rd::k_col_reg<8, false, kColLs[0], false, false, true>,
+      (const void *)rd::k_col_reg<8, true, kColLs[0], false, false, true>,
+      (const void *)rd::k_col_reg<8, false, kColLs[1], false, false, true>,
+      (const void *)rd::k_col_reg<8, true, kColLs[1], false, false, true>,
+      (const void *)rd::k_col_reg<8, false, kColLs3[0], false, false, true>,
+      (const void *)rd::k_col_reg<8, true, kColLs3[0], false, false, true>,
+      (const void *)rd::k_col_reg<8, false, kColLs3[1], false, false, true>,
+      (const void *)rd::k_col_reg<8, true, kColLs3[1], false, false, true>};
   for (const void *f : fs)
     HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return RDCU_OK;
