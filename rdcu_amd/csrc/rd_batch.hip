@@ -119,7 +119,11 @@ int allow_big_lds_b() {
         (const void *)rd::k_rowf_b<8, kRowLsB[2]>, (const void *)rd::k_col_b<8, kColLsB[0]>,
         (const void *)rd::k_col_b<8, kColLsB[1]>, (const void *)rd::k_col_b<8, kColLsB[2]>,
         (const void *)rd::k_rowf_b<8, kRowLsB3[0]>, (const void *)rd::k_rowf_b<8, kRowLsB3[1]>,
-        (const void *)rd::k_col_b<8, kColLsB3[0]>, (const void *)rd::k_col_b<8, kColLsB3[1]>})
+        (const void *)rd::k_col_b<8, kColLsB3[0]>, (const void *)rd::k_col_b<8, kColLsB3[1]>,
+        (const void *)rd::k_col_b<8, 0, true>, (const void *)rd::k_col_b<8, -1, true>,
+        (const void *)rd::k_col_b<8, kColLsB[0], true>, (const void *)rd::k_col_b<8, kColLsB[1], true>,
+        (const void *)rd::k_col_b<8, kColLsB[2], true>, (const void *)rd::k_col_b<8, kColLsB3[0], true>,
+        (const void *)rd::k_col_b<8, kColLsB3[1], true>})
     HIPCHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return RDCU_OK;
 }
@@ -135,6 +139,13 @@ void go_rowf_b(rdcu_batch *b, const BatchClass &cl, dim3 g, dim3 bl, size_t lds,
 template <int E, int LS>
 void go_col_b(rdcu_batch *b, const BatchClass &cl, dim3 g, dim3 bl, size_t lds, int per,
               const rd::ColGeom &cg, int fpw, int check) {
+  if constexpr (E == 8) {
+    if (fpw > 1) {  // several factors per block: their forward spectrum in registers
+      hipLaunchKernelGGL((rd::k_col_b<E, LS, true>), g, bl, lds, b->ls, b->d_inst + cl.off, cl.cnt,
+                         per, cl.c0->p0, cg, 1, fpw, check);
+      return;
+    }
+  }
   hipLaunchKernelGGL((rd::k_col_b<E, LS>), g, bl, lds, b->ls, b->d_inst + cl.off, cl.cnt, per,
                      cl.c0->p0, cg, 1, fpw, check);
 }
@@ -178,7 +189,10 @@ void launch_col_b(rdcu_batch *b, const BatchClass &cl, int tc, int fpw, int chec
   const long total = (long)cl.cnt * per;
   if (total == 0) return;
   const long g = (total + 7) / 8 * 8;  // one item per block, a multiple of the 8 XCDs
-  const size_t lds = (size_t)(cg.L + tc * rd::padded_len(cg.L) + tc * cg.L) * sizeof(cx) +
+  // the kept forward spectrum (tc*L) only for several factors per block with
+  // 16 values per thread; 8 per thread keep it in registers (go_col_b)
+  const bool keep = fpw > 1 && E != 8;
+  const size_t lds = (size_t)(cg.L + tc * rd::padded_len(cg.L) + (keep ? tc * cg.L : 0)) * sizeof(cx) +
                      (size_t)cg.L * sizeof(double);
   const dim3 gd((unsigned)g), bl(tc * T);
   if constexpr (E == 8) {

@@ -906,7 +906,8 @@ __global__ __launch_bounds__(256) void k_rowf_b(const Inst *__restrict__ in, int
 // column pass costs registers it does not have).  The grid is a multiple of
 // 8 and items are ordered so that each XCD takes a contiguous range of
 // logical blocks: neighbouring column tiles of one instance share its L2.
-template <int E, int LS = 0>
+// KR as for col_body (blocks with several factors, 8 values per thread).
+template <int E, int LS = 0, bool KR = false>
 __global__ __launch_bounds__(256) void k_col_b(const Inst *__restrict__ in, int ninst, int per,
                                                FftPlan p, ColGeom cg, int mode, int fpw,
                                                int check) {
@@ -915,7 +916,8 @@ __global__ __launch_bounds__(256) void k_col_b(const Inst *__restrict__ in, int 
   const int i = (int)(l / per);
   DevState *st = in[i].st;
   if (check && !attempt_active(st)) return;
-  col_body<E, false, LS>((int)(l - (long)i * per), in[i].g, p, cg, in[i].b, in[i].fl, mode, fpw);
+  col_body<E, false, LS, false, false, KR>((int)(l - (long)i * per), in[i].g, p, cg, in[i].b,
+                                           in[i].fl, mode, fpw);
 }
 
 // The x-inverse pass takes one (instance, row) item per block: it is the
