@@ -492,9 +492,10 @@ void go_col(rdcu_ctx *c, dim3 g, dim3 b, size_t lds, const rd::FftPlan &p, const
                          cg, c->st, check, c->bufs, fl, mode, fpw);
       return;
     }
-    // the 3-D y pass with the factors without k_z: their forward spectrum
-    // stays in registers (col_kr)
-    if (mode == 3) {
+    // several factors per block (the 3-D y pass with the factors without
+    // k_z, or the multiply pass with fpw > 1): their forward spectrum stays
+    // in registers (col_kr)
+    if (mode == 3 || (mode == 1 && fpw > 1)) {
       hipLaunchKernelGGL((rd::k_col_reg<E, NT, LS, false, false, true>), g, b, lds, c->stream,
                          c->grid, p, cg, c->st, check, c->bufs, fl, mode, fpw);
       return;
@@ -514,7 +515,8 @@ bool col_tg(int L, int E) { return E == 8 && L >= kColTgMin && !odd_len(L); }
 // wavenumbers, and with the kept spectrum when a block takes several
 // factors, fpw > 1).
 // The strided kernel keeps a multi-factor forward spectrum in registers
-// (k_col_reg<..., KR>) for the 3-D y pass (mode 3) with 8 values per thread.
+// (k_col_reg<..., KR>) with 8 values per thread, except on lines long enough
+// to read their twiddles from global memory.
 bool col_kr(int L, int E) { return E == 8 && !col_tg(L, E); }
 
 size_t col_lds(int L, int tc, int E, bool mode1, int fpw) {
@@ -556,7 +558,7 @@ void launch_colr(rdcu_ctx *c, const rd::FftPlan &p, rd::ColGeom cg, long ngroups
   // factors in one block
   const size_t lds =
       mode == 3 ? col_lds(cg.L, cg.tc, E, true, col_kr(cg.L, E) ? 1 : std::max(1, c->zs_used.zf_max))
-                : col_lds(cg.L, cg.tc, E, mode == 1, fpw);
+                : col_lds(cg.L, cg.tc, E, mode == 1, col_kr(cg.L, E) ? 1 : fpw);
   if (c->grid.nt_store)
     go_col_l<E, true>(c, g, dim3(cg.tc * T), lds, p, cg, check, fl, mode, fpw);
   else
@@ -988,7 +990,7 @@ void build_jit(rdcu_ctx *c) {
 // Whether every strided-pass launch with tc columns per block fits the LDS
 // of a CU (axis 0 keeps the forward spectrum; in 3-D axis 1 does not).
 bool col_fits(const rdcu_ctx *c, int tc, int fpw) {
-  size_t lds = col_lds(c->Ns[0], tc, c->E0, true, fpw);
+  size_t lds = col_lds(c->Ns[0], tc, c->E0, true, col_kr(c->Ns[0], c->E0) ? 1 : fpw);
   if (c->ndim == 3) {
     const int zf = c->zs_used.zf_max;
     lds = std::max(lds, col_lds(c->Ns[1], tc, c->E1, zf > 0,
