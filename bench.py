@@ -117,6 +117,30 @@ def measured_traffic(kernel):
     return None
 
 
+ATTEMPT_PROFILE = os.path.join(ROOT, "profiles", "attempt_2d_cgle_512.md")
+
+
+def measured_attempt_traffic():
+    """HBM bytes of one DP45 attempt at 512^2 from the committed PMC table of
+    the attempt profile (scaled MB per kernel position; the strided row
+    stands for its six launches), or None without it."""
+    if not os.path.exists(ATTEMPT_PROFILE):
+        return None
+    total, seen = 0.0, False
+    for line in open(ATTEMPT_PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[4] == "scaled MB":
+            seen = True
+            continue
+        if seen and len(cells) == 7 and cells[0][:1].isdigit():
+            try:
+                mb = float(cells[4])
+            except ValueError:
+                continue
+            total += 6 * mb if "," in cells[0] else mb
+    return total * 1e6 if seen else None
+
+
 def copy_calibration(mib, reps):
     """A plain device-to-device copy kernel (torch's elementwise copy) of a known
     size: the FETCH_SIZE / WRITE_SIZE counters it reads, against its exact
@@ -515,7 +539,8 @@ def main():
     B_att = attempt_bytes(2, M, Ns)
     result["attempt"] = {"bytes": B_att, "us": el / attempts * 1e6,
                          "achieved_GBps": B_att * attempts / el / 1e9,
-                         "frac": B_att * attempts / el / PEAK_HBM}
+                         "frac": B_att * attempts / el / PEAK_HBM,
+                         "traffic": measured_attempt_traffic() if (nd, G) == (2, 512) else None}
 
     if args.cpu_seconds > 0:
         result["cpu_baseline"] = cpu_leg(prob, y0, gname, args.warmup, attempts, accepted / el,

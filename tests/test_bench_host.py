@@ -65,3 +65,12 @@ def test_attempt_bytes_match_the_per_kernel_listing():
     A, Fz = 16 * N, 16 * N
     assert bench.attempt_bytes(2, 6, [256] * 3) - bench.attempt_bytes(2, 6, [256, 256 * 256]) \
         == 6 * (2 * A + 2 * Fz)
+
+
+def test_attempt_traffic_comes_from_the_counter_table():
+    """The attempt's measured HBM bytes are the sum of the PMC table's scaled
+    MB (the strided row six times), within 5% of the algorithmic bytes."""
+    t = bench.measured_attempt_traffic()
+    assert t is not None
+    assert abs(t / 1e6 - 350.6) < 0.5
+    assert 1.0 <= t / bench.attempt_bytes(2, 4, [512, 512]) < 1.05
