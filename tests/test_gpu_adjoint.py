@@ -75,6 +75,28 @@ def test_gradients_match_oracle_adjoint_and_tangent_linear(hlib, olib):
             assert float(gcg @ dc + gyg @ dy) == pytest.approx(dL, rel=1e-8)
 
 
+@pytest.mark.parametrize("store", ["1", "0"])
+def test_3d_gradients_match_oracle_adjoint(hlib, olib, monkeypatch, store):
+    """3-D CGLE (16x24x32): the reverse sweep's seven stage spectra run as one
+    problem of 7n fields, whose factor list is split into the factors without
+    k_z (the y pass) and the others (the z pass); gradients against the
+    oracle's adjoint, with the stages stored and recomputed."""
+    monkeypatch.setenv("RDCU_ADJ_STORE", store)
+    prob = Problem(2, [16, 24, 32], [10.0, 20.0, 30.0], cgle_terms(3, 2.0, 0.8))
+    y0 = glibc_rand_ic(2, prob.N, seed=6, A=1.0)
+    t_out = np.array([0.05, 0.1])
+    print(f"case 3-D gradients vs oracle store={store}", flush=True)
+    g, coef, target, hs = fit_problem(prob, y0, t_out, 0.1, hlib, 1e-10)
+    Lg, gcg, gyg = g.loss_grad(y0, 0.0, hs, t_out, target)
+    g.close()
+    assert hs.size > 3
+    with Integrator(olib, prob, y0, atol=1e-10, rtol=1e-10) as o:
+        Lo, gco, gyo = o.loss_grad(y0, 0.0, hs, t_out, target)
+    assert Lg == pytest.approx(Lo, rel=1e-12)
+    assert np.abs(gcg - gco).max() <= 1e-10 * np.abs(gco).max()
+    assert np.abs(gyg - gyo).max() <= 1e-10 * np.abs(gyo).max()
+
+
 def test_adjoint_config_vs_central_differences(hlib, olib):
     """2-D CGLE 256x256, L=100, t=10, 20 outputs, rtol=atol=1e-10, target from
     coefficients with 5% relative noise: every coefficient gradient against
